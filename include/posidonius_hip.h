@@ -1,0 +1,263 @@
+/*
+ * posidonius_hip.h -- C ABI of the MI355X WHFast ensemble integrator.
+ *
+ * The reference exposes its integration step through the Rust trait
+ * `Integrator` (src/integrator/mod.rs:16-26), implemented by `WHFast`
+ * (src/integrator/whfast.rs:98-305).  That trait is object-safe and
+ * single-system; this ABI is the same contract lifted to an ensemble of
+ * independent systems that live in device memory behind an opaque handle.
+ * A future Rust `impl Integrator for WHFastHip` binds these symbols one to
+ * one (see INTEGRATION.md for the cgo/ctypes/Rust-FFI stubs).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative POS_E_* code on
+ *     failure; pos_last_error() returns a thread-local message for the last
+ *     failure on the calling thread.
+ *   - The caller owns every host buffer; the handle owns every device buffer
+ *     and one HIP stream.  A handle is not thread-safe.
+ *   - All arithmetic is IEEE binary64.  Units are the reference's Mercury
+ *     units: AU, day, solar mass, G = K2 (src/constants.rs:59-61).
+ *   - Physics failures never abort: the reference's panic!() paths
+ *     (src/particles/universe.rs:218-238) and warnings
+ *     (src/integrator/whfast.rs:389-391, 702-707) become per-system status
+ *     bits, and a system with a POS_STATUS_FATAL bit set stops integrating.
+ */
+#ifndef POSIDONIUS_HIP_H
+#define POSIDONIUS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define POS_ABI_VERSION 1
+
+/* Largest body count per system.  The reference caps a universe at
+ * MAX_PARTICLES = 10 (src/constants.rs:3); the uneven-size ensemble needs 12.
+ * A system is mapped to one lane group of at most 16 lanes. */
+#define POS_MAX_BODIES 16
+
+/* ---- return codes ------------------------------------------------------ */
+#define POS_OK                0
+#define POS_E_INVALID        -1   /* bad argument or unsupported case      */
+#define POS_E_HIP            -2   /* a HIP runtime call failed             */
+#define POS_E_NO_DEVICE      -3   /* no usable gfx950 device               */
+#define POS_E_UNSUPPORTED    -4   /* feature not available on the device   */
+#define POS_E_LIMIT          -5   /* size beyond a documented maximum      */
+
+/* ---- WHFast alternative coordinates (whfast.rs:91-95) ------------------- */
+#define POS_COORD_JACOBI                  0
+#define POS_COORD_DEMOCRATIC_HELIOCENTRIC 1
+#define POS_COORD_WHDS                    2
+
+/* ---- general relativity implementation (general_relativity.rs:40-45) --- */
+#define POS_GR_KIDDER1995    0
+#define POS_GR_ANDERSON1975  1
+#define POS_GR_NEWHALL1983   2
+#define POS_GR_DISABLED      3
+
+/* ---- consider_effects (universe.rs:40-47), one bitmask per system ------ *
+ * Values as left by Universe::new after disable_unnecessary_effects
+ * (universe.rs:699-764). */
+#define POS_EFFECT_TIDES                (1u << 0)
+#define POS_EFFECT_ROTATIONAL_FLATTENING (1u << 1)
+#define POS_EFFECT_GENERAL_RELATIVITY   (1u << 2)
+#define POS_EFFECT_DISK                 (1u << 3)
+#define POS_EFFECT_WIND                 (1u << 4)
+#define POS_EFFECT_EVOLUTION            (1u << 5)
+
+/* ---- per-body effect roles (TidesEffect, RotationalFlatteningEffect,
+ *      GeneralRelativityEffect, WindEffect enums) ------------------------- */
+#define POS_ROLE_TIDES_CENTRAL   (1u << 0)
+#define POS_ROLE_TIDES_ORBITING  (1u << 1)
+#define POS_ROLE_FLAT_CENTRAL    (1u << 2)
+#define POS_ROLE_FLAT_ORBITING   (1u << 3)
+#define POS_ROLE_GR_CENTRAL      (1u << 4)
+#define POS_ROLE_GR_ORBITING     (1u << 5)
+#define POS_ROLE_WIND            (1u << 6)
+
+/* ---- evolution models (src/effects/evolution.rs:9-17) ------------------ */
+#define POS_EVO_NON_EVOLVING              0
+#define POS_EVO_LECONTE2011               1
+#define POS_EVO_BARAFFE1998               2
+#define POS_EVO_BARAFFE2015               3
+#define POS_EVO_LECONTE_CHABRIER2013      4  /* without dynamical tides */
+#define POS_EVO_LECONTE_CHABRIER2013_DYN  5  /* with dynamical tides    */
+#define POS_EVO_BOLMONT_MATHIS2016        6
+#define POS_EVO_GALLET_BOLMONT2017        7
+
+/* ---- per-system status bits -------------------------------------------- */
+#define POS_STATUS_COMPLETED          (1u << 0) /* t + dt > time_limit (whfast.rs:300-302) */
+#define POS_STATUS_ROCHE              (1u << 1) /* universe.rs:225-228 */
+#define POS_STATUS_COLLISION          (1u << 2) /* universe.rs:230-233 */
+#define POS_STATUS_EJECTION           (1u << 3) /* universe.rs:234-237 */
+#define POS_STATUS_ZERO_INERTIA       (1u << 4) /* particles/common.rs:5-7 */
+#define POS_STATUS_MIDPOINT_NOT_CONVERGED (1u << 5) /* whfast.rs:389-391 (warning) */
+#define POS_STATUS_TIMESTEP_WARNING   (1u << 6) /* whfast.rs:702-707 (warning) */
+#define POS_STATUS_KEPLER_BISECTION   (1u << 7) /* whfast.rs:776-808 fallback used */
+#define POS_STATUS_ITERATION_CAP      (1u << 8) /* a loop the reference leaves unbounded hit its cap */
+#define POS_STATUS_FATAL (POS_STATUS_ROCHE | POS_STATUS_COLLISION | POS_STATUS_EJECTION | \
+                          POS_STATUS_ZERO_INERTIA | POS_STATUS_ITERATION_CAP)
+
+/*
+ * Ensemble description: what the reference's case.json holds for each
+ * universe (Universe, Particle and WHFast fields, universe.rs:50-63,
+ * particle.rs:16-52, whfast.rs:98-120), flattened to structure-of-arrays.
+ * Bodies of system s are body_offset[s] .. body_offset[s+1]-1, in the order
+ * of the reference's particle array.  Vector fields hold 3 doubles per body
+ * (x, y, z).  Pointers marked "optional" may be NULL.
+ */
+typedef struct pos_ensemble_desc {
+    int32_t n_systems;
+    const int32_t *body_offset;              /* [n_systems + 1]                      */
+
+    /* per system */
+    const int32_t  *coordinates;             /* POS_COORD_*                          */
+    const uint32_t *effects;                 /* POS_EFFECT_* bitmask                 */
+    const int32_t  *gr_implementation;       /* POS_GR_*                             */
+    const double   *time_step;               /* dt (days)                            */
+    const double   *time_limit;              /* universe.time_limit                  */
+    const double   *current_time;            /* optional: WHFast.current_time, 0     */
+    const double   *historic_snapshot_period;/* optional: default +inf (never)       */
+    const double   *last_historic_snapshot_time; /* optional: default -1 (fresh)     */
+    const int64_t  *current_iteration;       /* optional: default 0                  */
+
+    /* per body */
+    const double   *mass;                    /* M_sun                                */
+    const double   *radius;                  /* AU                                   */
+    const double   *radius_of_gyration_2;
+    const double   *moment_of_inertia;       /* optional: default m*rg2*R^2          */
+    const double   *position;                /* [3*nb] inertial (barycentric), AU    */
+    const double   *velocity;                /* [3*nb] inertial, AU/day              */
+    const double   *spin;                    /* optional [3*nb]; default L/I         */
+    const double   *angular_momentum;        /* [3*nb] spin angular momentum         */
+    const double   *heliocentric_velocity;   /* optional [3*nb]; only the host's value
+                                                is read (stale before the first step,
+                                                universe.rs:335-337)                 */
+    const uint32_t *roles;                   /* POS_ROLE_* bitmask                   */
+    const double   *tides_dissipation_factor;       /* CTL sigma (ConstantTimeLagParameters) */
+    const double   *tides_dissipation_factor_scale;
+    const double   *tides_love_number;
+    const double   *flattening_love_number;  /* OblateSpheroidParameters.love_number */
+    const double   *wind_k_factor;           /* optional                             */
+    const double   *wind_rotation_saturation;/* optional                             */
+
+    /* integrator state (optional, zero when NULL): compensated-summation
+     * errors, whfast.rs:117-119 */
+    const double   *velocity_error;          /* [3*nb] */
+    const double   *angular_momentum_error;  /* [3*nb] */
+
+    /* evolution tables (optional; all bodies non-evolving when NULL).
+     * Rows of body b are evolver_offset[b] .. evolver_offset[b+1]-1 of the
+     * flat time/radius/rg2/love arrays; time is relative to initial_time as
+     * in case.json (evolution.rs:426-427). */
+    const int32_t  *evolution_type;          /* [nb] POS_EVO_*                       */
+    const int32_t  *evolver_offset;          /* [nb + 1]                             */
+    const double   *evolver_time;
+    const double   *evolver_radius;
+    const double   *evolver_radius_of_gyration_2;
+    const double   *evolver_love_number;
+    const int32_t  *evolver_left_index;      /* [nb] Evolver.left_index              */
+} pos_ensemble_desc;
+
+/*
+ * Mutable state of an ensemble, read by pos_get_state and written by
+ * pos_set_state.  Every pointer is optional; NULL fields are skipped.
+ * Per-body arrays have the ensemble's total body count (x3 for vectors).
+ */
+typedef struct pos_state {
+    double   *position;              /* [3*nb] inertial                       */
+    double   *velocity;              /* [3*nb] inertial                       */
+    double   *acceleration;          /* [3*nb] gravity, universe.rs:300-302   */
+    double   *angular_momentum;      /* [3*nb]                                */
+    double   *spin;                  /* [3*nb]                                */
+    double   *velocity_error;        /* [3*nb]                                */
+    double   *angular_momentum_error;/* [3*nb]                                */
+    double   *radius;                /* [nb] (changes only for evolving bodies)*/
+    double   *radius_of_gyration_2;  /* [nb]                                  */
+    double   *moment_of_inertia;     /* [nb]                                  */
+    double   *current_time;          /* [n_systems]                           */
+    double   *last_historic_snapshot_time; /* [n_systems]                     */
+    int64_t  *current_iteration;     /* [n_systems]                           */
+    uint32_t *status;                /* [n_systems] POS_STATUS_* bits         */
+} pos_state;
+
+/* Launch mapping chosen by the autotuner or by the caller. */
+#define POS_MAPPING_AUTO        0   /* autotuned (pos_tune) or heuristic    */
+#define POS_MAPPING_PLAIN       1   /* one system per wavefront, one step per launch */
+#define POS_MAPPING_PACKED      2   /* lane groups, K steps per launch      */
+
+typedef struct pos_options {
+    int32_t device;            /* HIP device ordinal                          */
+    int32_t mapping;           /* POS_MAPPING_*                               */
+    int32_t group_size;        /* lanes per system for PACKED (0 = smallest power of two >= N) */
+    int32_t steps_per_launch;  /* K for PACKED (0 = default 64)               */
+    int32_t use_graph;         /* capture chunk sequences in a hipGraph       */
+    int32_t reserved[3];
+} pos_options;
+
+typedef struct pos_ens pos_ens;   /* opaque */
+
+/* ABI version this library was built with. */
+int32_t pos_abi_version(void);
+
+/* Thread-local message for the last failing call on this thread. */
+const char *pos_last_error(void);
+
+/* Build an ensemble on the device.  Replaces WHFast::new + Universe::new
+ * (whfast.rs:143-165, universe.rs:73-175) and, for systems whose
+ * current_time is 0, Integrator::initialize_physical_values
+ * (whfast.rs:226-233): spin = L/I and Roche radii (universe.rs:177-196). */
+int32_t pos_create(const pos_ensemble_desc *desc, const pos_options *opts, pos_ens **out);
+
+/* Advance every system by n_steps calls of WHFast::iterate
+ * (whfast.rs:235-305).  Systems whose status has POS_STATUS_COMPLETED or a
+ * fatal bit are left unchanged.  Asynchronous with respect to the host;
+ * pos_synchronize or pos_get_state waits for completion. */
+int32_t pos_step(pos_ens *ens, int64_t n_steps);
+
+/* Wait for all work queued on the handle's stream. */
+int32_t pos_synchronize(pos_ens *ens);
+
+/* Copy state device -> host (blocking) / host -> device. */
+int32_t pos_get_state(pos_ens *ens, pos_state *out);
+int32_t pos_set_state(pos_ens *ens, const pos_state *in);
+
+/* Integrator::set_time_limit / set_snapshot_periods (whfast.rs:187-224),
+ * applied to every system; a value <= 0 leaves the field unchanged. */
+int32_t pos_set_time_limit(pos_ens *ens, double time_limit);
+int32_t pos_set_historic_snapshot_period(pos_ens *ens, double period);
+
+/* Sizes of the ensemble. */
+int32_t pos_get_counts(const pos_ens *ens, int32_t *n_systems, int32_t *n_bodies);
+
+/* The launch mapping currently in use (after pos_tune or the heuristic). */
+int32_t pos_get_options(const pos_ens *ens, pos_options *out);
+
+/* Autotuner: times candidate (group size, steps per launch) mappings on a
+ * copy of the ensemble for `probe_steps` steps each, keeps the fastest, and
+ * restores the state.  Writes the chosen mapping to *chosen (optional). */
+int32_t pos_tune(pos_ens *ens, int32_t probe_steps, pos_options *chosen);
+
+/* Operator-level entry points: the two sub-boundaries the reference
+ * benchmarks directly (benches/simulation.rs:127, 138).  They evaluate at
+ * the current state without advancing time and copy results to the host.
+ *   pos_gravity_acceleration     Universe::gravity_calculate_acceleration
+ *                                (universe.rs:198-303) -> out_acc[3*nb]
+ *   pos_additional_effects       inertial_to_heliocentric +
+ *                                Universe::calculate_additional_effects
+ *                                (universe.rs:318-351, 428-538) ->
+ *                                out_acc[3*nb], out_dangular_momentum_dt[3*nb]
+ * Neither modifies the ensemble state.  Status bits raised by the gravity
+ * checks are OR-ed into out_status[n_systems] (optional). */
+int32_t pos_gravity_acceleration(pos_ens *ens, double *out_acc, uint32_t *out_status);
+int32_t pos_additional_effects(pos_ens *ens, double *out_acc, double *out_dangular_momentum_dt);
+
+/* Release device memory and the stream. */
+void pos_destroy(pos_ens *ens);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POSIDONIUS_HIP_H */

@@ -1,0 +1,144 @@
+// oracle_capi.cpp -- extern "C" entry points of the CPU oracle (liboracle.so).
+//
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py through ctypes.  It consumes the same
+// pos_ensemble_desc / pos_state structures as the product library so that
+// both sides read identical inputs.
+#include <atomic>
+#include <chrono>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "posidonius_oracle.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class R> void store_state(const oracle::System<R> &S, int b0, int s, pos_state *o) {
+    using oracle::val;
+    for (int i = 0; i < S.n; ++i) {
+        const int b = b0 + i;
+        const auto &P = S.p[i];
+        auto put3 = [&](double *dst, const oracle::V3<R> &v) {
+            if (!dst) return;
+            dst[3 * b] = val(v.x); dst[3 * b + 1] = val(v.y); dst[3 * b + 2] = val(v.z);
+        };
+        put3(o->position, P.x);
+        put3(o->velocity, P.v);
+        put3(o->acceleration, P.a_grav);
+        put3(o->angular_momentum, P.L);
+        put3(o->spin, P.spin);
+        put3(o->velocity_error, S.err_v[i]);
+        put3(o->angular_momentum_error, S.err_L[i]);
+        if (o->radius) o->radius[b] = val(P.radius);
+        if (o->radius_of_gyration_2) o->radius_of_gyration_2[b] = val(P.rg2);
+        if (o->moment_of_inertia) o->moment_of_inertia[b] = val(P.inertia);
+    }
+    if (o->current_time) o->current_time[s] = val(S.t);
+    if (o->last_historic_snapshot_time) o->last_historic_snapshot_time[s] = S.last_hist;
+    if (o->current_iteration) o->current_iteration[s] = S.iteration;
+    if (o->status) o->status[s] = S.status;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *oracle_last_error(void) { return g_err.c_str(); }
+
+// Integrate every system for up to n_steps calls of WHFast::iterate (fewer if
+// it completes or freezes) on n_threads host threads, one system at a time
+// per thread.  If out_midpoint_iterations is non-NULL it receives, per system,
+// the number of additional-effects evaluations executed.  Returns 0 or -1.
+int oracle_run(const pos_ensemble_desc *d, int64_t n_steps, int32_t n_threads, pos_state *out,
+               int64_t *out_steps_done, int64_t *out_midpoint_iterations) {
+    const int ns = d->n_systems;
+    std::atomic<int> next{0};
+    std::atomic<int> failed{-1};
+    std::string first_err;
+    auto worker = [&]() {
+        oracle::System<double> *S = new oracle::System<double>();
+        for (;;) {
+            const int s = next.fetch_add(1);
+            if (s >= ns) break;
+            std::string err;
+            if (!oracle::load_system(d, s, *S, &err)) {
+                int expected = -1;
+                if (failed.compare_exchange_strong(expected, s)) first_err = err;
+                continue;
+            }
+            if (S->t == 0.) oracle::initialize_physical_values(*S);
+            int64_t k = 0;
+            while (k < n_steps) {
+                ++k;
+                if (!oracle::iterate(*S)) break;
+            }
+            store_state(*S, d->body_offset[s], s, out);
+            if (out_steps_done) out_steps_done[s] = k;
+            if (out_midpoint_iterations) out_midpoint_iterations[s] = S->midpoint_iterations;
+        }
+        delete S;
+    };
+    const int nt = n_threads < 1 ? 1 : n_threads;
+    if (nt == 1) {
+        worker();
+    } else {
+        std::vector<std::thread> pool;
+        for (int i = 0; i < nt; ++i) pool.emplace_back(worker);
+        for (auto &t : pool) t.join();
+    }
+    if (failed.load() >= 0) {
+        g_err = "system " + std::to_string(failed.load()) + ": " + first_err;
+        return -1;
+    }
+    return 0;
+}
+
+// Operator-level evaluations at the current state (no time advance).
+// Universe::gravity_calculate_acceleration with the WHFast ignore rule of
+// the system's coordinates (whfast.rs:263-267).
+int oracle_gravity(const pos_ensemble_desc *d, double *out_acc, uint32_t *out_status) {
+    oracle::System<double> *S = new oracle::System<double>();
+    for (int s = 0; s < d->n_systems; ++s) {
+        std::string err;
+        if (!oracle::load_system(d, s, *S, &err)) { g_err = err; delete S; return -1; }
+        if (S->t == 0.) oracle::initialize_physical_values(*S);
+        const int ignore = S->coord == POS_COORD_JACOBI ? oracle::IGNORE_WHFAST_ONE : oracle::IGNORE_WHFAST_TWO;
+        oracle::gravity_calculate_acceleration(*S, ignore);
+        const int b0 = d->body_offset[s];
+        for (int i = 0; i < S->n; ++i) {
+            out_acc[3 * (b0 + i)] = S->p[i].a_grav.x;
+            out_acc[3 * (b0 + i) + 1] = S->p[i].a_grav.y;
+            out_acc[3 * (b0 + i) + 2] = S->p[i].a_grav.z;
+        }
+        if (out_status) out_status[s] |= S->status;
+    }
+    delete S;
+    return 0;
+}
+
+// inertial_to_heliocentric + calculate_additional_effects(t, evolution=false,
+// dangular_momentum_dt=integrate_spin, accelerations=true).
+int oracle_additional_effects(const pos_ensemble_desc *d, double *out_acc, double *out_dLdt) {
+    oracle::System<double> *S = new oracle::System<double>();
+    for (int s = 0; s < d->n_systems; ++s) {
+        std::string err;
+        if (!oracle::load_system(d, s, *S, &err)) { g_err = err; delete S; return -1; }
+        if (S->t == 0.) oracle::initialize_physical_values(*S);
+        const bool integrate_spin = S->tides || S->rf || S->evolution || (S->gr && S->gr_impl == POS_GR_KIDDER1995);
+        oracle::inertial_to_heliocentric(*S);
+        oracle::calculate_additional_effects(*S, S->t, false, integrate_spin, true);
+        const int b0 = d->body_offset[s];
+        for (int i = 0; i < S->n; ++i) {
+            const int b = b0 + i;
+            out_acc[3 * b] = S->p[i].a_add.x; out_acc[3 * b + 1] = S->p[i].a_add.y; out_acc[3 * b + 2] = S->p[i].a_add.z;
+            out_dLdt[3 * b] = S->p[i].dLdt.x; out_dLdt[3 * b + 1] = S->p[i].dLdt.y; out_dLdt[3 * b + 2] = S->p[i].dLdt.z;
+        }
+    }
+    delete S;
+    return 0;
+}
+
+}  // extern "C"
