@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Benchmark of the WHFast + full-effects step on one MI355X.
+
+Workload (default): the uneven-size ensemble of 65,536 independent systems,
+each a 0.09 M_sun star with 1-11 Earth-mass planets in compact chains (2-12
+bodies), with constant-time-lag tides, rotational flattening, Kidder 1995 GR
+and spin integration, in FP64.  A "step" is one WHFast iterate of every
+system; the metric is system-steps per second of the batched (packed,
+multi-step, graph-replayed, autotuned) kernel, with the plain mapping (one
+system per wavefront, one step per launch) measured beside it.
+
+Printed: one JSON line on stdout.  Progress goes to stderr.
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--systems B]
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic FP64 operations (+ - * / sqrt, each counted once; nothing is
+# fused) per system-step, by body count, for the uneven-size ensemble.  They
+# are the means measured by the oracle's FlopCount instantiation over the
+# workload's own systems (256 systems x 100 steps, seed 1002), i.e. for the
+# iteration counts the data actually take; tests/test_flop_table.py re-derives
+# them.
+FLOPS_PER_SYSTEM_STEP = {2: 5978, 3: 11356, 4: 16747, 5: 22177, 6: 27614, 7: 33118, 8: 38616,
+                         9: 44177, 10: 49766, 11: 55408, 12: 61076}
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix), AMD specification
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """Two HIP events recorded on an explicit stream (the library's own)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.e0, self.e1 = C.c_void_p(), C.c_void_p()
+        for e in (self.e0, self.e1):
+            if self.hip.hipEventCreate(C.byref(e)) != 0:
+                raise RuntimeError("hipEventCreate failed")
+
+    def record(self, which, stream):
+        if self.hip.hipEventRecord(self.e0 if which == 0 else self.e1, C.c_void_p(stream)) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def elapsed_ms(self):
+        self.hip.hipEventSynchronize(self.e1)
+        ms = C.c_float()
+        if self.hip.hipEventElapsedTime(C.byref(ms), self.e0, self.e1) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return float(ms.value)
+
+
+def timed_steps(ens, steps, events):
+    ens.synchronize()
+    stream = ens.stream()
+    t0 = time.perf_counter()
+    events.record(0, stream)
+    ens.step(steps)
+    events.record(1, stream)
+    ens.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, events.elapsed_ms()
+
+
+def cpu_baseline(spec, threads, target_s=12.0):
+    """The oracle (C++ restatement of the reference step) on the host cores:
+    one system per thread at a time, `threads` threads, a bounded sample."""
+    import numpy as np
+    from tests.oracle_runner import Oracle
+    oracle = Oracle()
+    n = threads * 16
+    sample = spec.subset(np.arange(n))
+    t0 = time.perf_counter()
+    oracle.run(sample, 20, threads=threads, fields=("status",))
+    calib = time.perf_counter() - t0
+    steps = max(20, int(20 * target_s / max(calib, 1e-3)))
+    t0 = time.perf_counter()
+    out = oracle.run(sample, steps, threads=threads, fields=("status",))
+    wall = time.perf_counter() - t0
+    done = int(out["steps_done"].sum())
+    return {"value": done / wall, "unit": "system-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} systems of the same ensemble (first {n}), {steps} steps each, "
+                      f"{threads} threads, one system per thread at a time; {wall:.1f} s wall",
+            "build": "g++ -O3 -ffp-contract=off (oracle/Makefile)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=256)
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4])
+    ap.add_argument("--systems", type=int, default=None)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--group-size", type=int, default=0)
+    ap.add_argument("--steps-per-launch", type=int, default=0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: every system is independent, so the ensemble "
+              "shards by systems with no exchange; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from posidonius_amd import generators as gen
+    from posidonius_amd.ensemble import Ensemble
+
+    defaults = {1: 4096, 2: 65536, 3: 16384, 4: 1024}
+    n_sys = args.systems or defaults[args.config]
+    log(f"generating config {args.config} with {n_sys} systems")
+    spec = gen.CONFIGS[args.config](n_sys)
+    counts = spec.n_per_system()
+    flops_per_step = float(sum(FLOPS_PER_SYSTEM_STEP[int(n)] for n in counts))
+    events = HipEvents()
+
+    # ---- batched path -----------------------------------------------------
+    log("creating the batched ensemble")
+    ens = Ensemble(spec, mapping="packed", group_size=args.group_size,
+                   steps_per_launch=args.steps_per_launch, use_graph=True)
+    if not args.no_tune:
+        log("autotuning the launch mapping")
+        o = ens.tune(probe_steps=32)
+        log(f"chosen group_size={o.group_size} steps_per_launch={o.steps_per_launch}")
+    o = ens.options()
+    K = o.steps_per_launch
+    log(f"warmup {args.warmup} steps")
+    ens.step(args.warmup)
+    ens.synchronize()
+    log(f"timing {args.steps} steps")
+    wall, ev_ms = timed_steps(ens, args.steps, events)
+    st = ens.get_state(fields=("status",))
+    frozen = int(np.count_nonzero(st["status"] & 0x11e))
+    ens.close()
+    launches = math.ceil(args.steps / K)
+    value = n_sys * args.steps / wall
+    kernel_ms = ev_ms / launches
+    flops_per_launch = flops_per_step * min(K, args.steps)
+    achieved_tflops = flops_per_launch / (kernel_ms * 1e-3) / 1e12
+    log(f"batched: {value:.4g} system-steps/s, {wall / args.steps * 1e3:.3f} ms/step, "
+        f"kernel {kernel_ms:.3f} ms/launch, {achieved_tflops:.2f} TFLOP/s algorithmic")
+
+    # ---- plain path ---------------------------------------------------------
+    plain = None
+    if not args.no_plain:
+        log("creating the plain ensemble (one system per wavefront, one step per launch)")
+        pens = Ensemble(spec, mapping="plain")
+        pens.step(1)
+        pwall, pms = timed_steps(pens, args.plain_steps, events)
+        pens.close()
+        prate = n_sys * args.plain_steps / pwall
+        plain = {"value": prate, "unit": "system-steps/s", "ms_per_step": pwall / args.plain_steps * 1e3,
+                 "steps": args.plain_steps, "mapping": "one system per wavefront, one step per launch, "
+                                                      "state in HBM between steps"}
+        log(f"plain: {prate:.4g} system-steps/s ({pwall / args.plain_steps * 1e3:.2f} ms/step)")
+
+    # ---- CPU baseline ---------------------------------------------------------
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        log(f"CPU baseline on {threads} host threads")
+        cpu = cpu_baseline(spec, threads)
+        log(f"cpu: {cpu['value']:.4g} system-steps/s")
+
+    mean_n = float(counts.mean())
+    result = {
+        "metric": "ensemble system-steps/s FP64 full effects (batched vs plain GPU); adjoint/fwd cost",
+        "value": value,
+        "unit": "system-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": {1: "uniform ensemble: 4096 star + 2 planet systems",
+                         2: "uneven-size ensemble: 65536 systems of 2-12 bodies (1 star + 1-11 planets)",
+                         3: "binary-host ensemble: 16384 systems, 2 stars + 1-3 circumbinary planets",
+                         4: "adjoint workload forward pass: 1024 star + 3 planet systems"}[args.config]
+                        if n_sys == defaults[args.config] else f"config {args.config} with {n_sys} systems",
+            "systems": n_sys, "mean_bodies": mean_n, "effects": "CTL tides + oblate flattening + Kidder GR + spin",
+            "coordinates": "Jacobi" if args.config == 3 else "DemocraticHeliocentric",
+            "group_size": o.group_size, "steps_per_launch": K, "hip_graph": True,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "note": "FP64 compute bound (VALU; nothing in the step is a contraction, no MFMA is used); "
+                    "peak is MI355X FP64 78.6 TFLOP/s",
+            "achieved": achieved_tflops,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "whfast_steps_kernel",
+            "kernel_ms_per_launch": kernel_ms,
+            "flops_per_launch": flops_per_launch,
+            "flops_per_system_step_mean": flops_per_step / n_sys,
+        },
+        "cpu_baseline": cpu,
+        "plain_gpu": plain,
+        "batched_over_plain": (value / plain["value"]) if plain else None,
+        "frozen_systems": frozen,
+    }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -232,6 +232,11 @@ int32_t pos_set_historic_snapshot_period(pos_ens *ens, double period);
 /* Sizes of the ensemble. */
 int32_t pos_get_counts(const pos_ens *ens, int32_t *n_systems, int32_t *n_bodies);
 
+/* The HIP stream (hipStream_t) the handle launches on, so that a caller can
+ * order its own work or timing events against it.  No counterpart in the
+ * reference (which has no device). */
+int32_t pos_get_stream(const pos_ens *ens, void **stream);
+
 /* The launch mapping currently in use (after pos_tune or the heuristic). */
 int32_t pos_get_options(const pos_ens *ens, pos_options *out);
 

@@ -10,6 +10,7 @@
 #include <thread>
 #include <vector>
 
+#include "oracle_types.hpp"
 #include "posidonius_oracle.hpp"
 
 namespace {
@@ -136,6 +137,27 @@ int oracle_additional_effects(const pos_ensemble_desc *d, double *out_acc, doubl
             out_acc[3 * b] = S->p[i].a_add.x; out_acc[3 * b + 1] = S->p[i].a_add.y; out_acc[3 * b + 2] = S->p[i].a_add.z;
             out_dLdt[3 * b] = S->p[i].dLdt.x; out_dLdt[3 * b + 1] = S->p[i].dLdt.y; out_dLdt[3 * b + 2] = S->p[i].dLdt.z;
         }
+    }
+    delete S;
+    return 0;
+}
+
+// Algorithmic FP64 operation count (+ - * / sqrt) per system for n_steps
+// steps, from the FlopCount instantiation of the same restated step.
+int oracle_flops(const pos_ensemble_desc *d, int64_t n_steps, double *out_flops, int64_t *out_steps_done) {
+    oracle::System<oracle::FlopCount> *S = new oracle::System<oracle::FlopCount>();
+    for (int s = 0; s < d->n_systems; ++s) {
+        std::string err;
+        if (!oracle::load_system(d, s, *S, &err)) { g_err = err; delete S; return -1; }
+        if (oracle::val(S->t) == 0.) oracle::initialize_physical_values(*S);
+        oracle::FlopCount::count = 0;
+        int64_t k = 0;
+        while (k < n_steps) {
+            ++k;
+            if (!oracle::iterate(*S)) break;
+        }
+        out_flops[s] = (double)oracle::FlopCount::count;
+        if (out_steps_done) out_steps_done[s] = k;
     }
     delete S;
     return 0;

@@ -96,7 +96,12 @@ struct Evolver {
     int len = 0;
     int64_t left_index = 0;
 
-    int idx() const { return left_index > 10 ? (int)(left_index - 10) : 0; }
+    // evolution.rs:449-456; clamped to the last row, which a consistent
+    // left_index never reaches (the reference would panic on the slice).
+    int idx() const {
+        const int i0 = left_index > 10 ? (int)(left_index - 10) : 0;
+        return i0 < len ? i0 : len - 1;
+    }
 
     // tools.rs:840-860 linear_interpolation + 882-905 find_indices_around_target_value
     template <class R>

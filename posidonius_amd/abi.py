@@ -244,6 +244,8 @@ def _declare(lib):
     lib.pos_set_historic_snapshot_period.restype = C.c_int32
     lib.pos_get_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     lib.pos_get_counts.restype = C.c_int32
+    lib.pos_get_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+    lib.pos_get_stream.restype = C.c_int32
     lib.pos_get_options.argtypes = [C.c_void_p, C.POINTER(Options)]
     lib.pos_get_options.restype = C.c_int32
     lib.pos_tune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Options)]
@@ -259,7 +261,7 @@ def _declare(lib):
 EXPORTED_SYMBOLS = (
     "pos_abi_version", "pos_last_error", "pos_create", "pos_step", "pos_synchronize",
     "pos_get_state", "pos_set_state", "pos_set_time_limit", "pos_set_historic_snapshot_period",
-    "pos_get_counts", "pos_get_options", "pos_tune", "pos_gravity_acceleration",
+    "pos_get_counts", "pos_get_stream", "pos_get_options", "pos_tune", "pos_gravity_acceleration",
     "pos_additional_effects", "pos_destroy",
 )
 

@@ -1,0 +1,970 @@
+// posidonius_hip.hip -- kernels and the C ABI (include/posidonius_hip.h).
+//
+// Device layout: systems are reordered into "device order" (sorted by body
+// count, then by input index) so that the systems of one wavefront are
+// contiguous in memory.  Per-body vectors are stored component-major
+// ([3][n_bodies]), per-system scalars in plain arrays.  The host keeps the
+// permutation and applies it in pos_get_state / pos_set_state.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "whfast_step.hpp"
+
+using namespace posdev;
+
+// ===========================================================================
+// Kernels
+// ===========================================================================
+namespace {
+
+__device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, int wave) {
+    const int lane = threadIdx.x;
+    const int G = E.wave_group[wave];
+    const int g = lane / G;
+    const int k = lane - g * G;
+    const int count = E.wave_count[wave];
+    const bool gvalid = g < count;
+    const int s = E.wave_first[wave] + (gvalid ? g : 0);
+    B.lane = lane;
+    B.gbase = g * G;
+    B.k = k;
+    B.sys = s;
+    B.n = gvalid ? (E.body_offset[s + 1] - E.body_offset[s]) : 0;
+    B.valid = gvalid && k < B.n;
+    const int b0 = E.body_offset[s];
+    B.host = gvalid ? E.host[s] : 0;   // keeps gbase + host inside this wave's LDS rows
+    B.is_host = B.valid && (k == B.host);
+    B.body = b0 + (B.valid ? k : 0);
+    B.coord = E.coord[s];
+    B.eff = E.effects[s];
+    B.tides = B.eff & POS_EFFECT_TIDES;
+    B.rf = B.eff & POS_EFFECT_ROTATIONAL_FLATTENING;
+    B.gr = B.eff & POS_EFFECT_GENERAL_RELATIVITY;
+    B.wind = B.eff & POS_EFFECT_WIND;
+    B.evolution = B.eff & POS_EFFECT_EVOLUTION;
+    B.spin_on = B.tides || B.rf || B.evolution || B.gr;
+    B.dt = E.dt[s];
+    B.half_dt = 0.5 * B.dt;
+    B.T = E.time_limit[s];
+    B.t = E.t[s];
+    B.last_hist = E.last_hist[s];
+    B.hist_period = E.hist_period[s];
+    B.iteration = E.iteration[s];
+    B.status = E.status[s];
+    if (!B.valid) B.status |= POS_STATUS_COMPLETED;   // idle lanes never iterate
+    B.tw = E.timestep_warning[s];
+    const int b = B.body;
+    const int nb = E.n_bodies;
+    B.m = E.mass[b];
+    B.mg = B.m * K2;
+    B.R = E.radius[b];
+    B.rg2 = E.rg2[b];
+    B.I = E.inertia[b];
+    B.sigma = E.sigma[b];
+    B.k2 = E.k2[b];
+    B.k2f = E.k2f[b];
+    B.wk = E.wind_k[b];
+    B.wsat = E.wind_sat[b];
+    B.wsat2 = B.wsat * B.wsat;
+    B.roles = E.roles[b];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        B.x[q] = E.x[q * nb + b];
+        B.v[q] = E.v[q * nb + b];
+        B.L[q] = E.L[q * nb + b];
+        B.s[q] = E.spin[q * nb + b];
+        B.ev[q] = E.err_v[q * nb + b];
+        B.eL[q] = E.err_L[q * nb + b];
+        B.ag[q] = E.acc[q * nb + b];
+    }
+    B.s2 = p2(B.s[0]) + p2(B.s[1]) + p2(B.s[2]);
+    const int hb = b0 + B.host;
+    B.hm = E.mass[hb];
+    B.hmg = B.hm * K2;
+    B.hsigma = E.sigma[hb];
+    B.hk2 = E.k2[hb];
+    B.hk2f = E.k2f[hb];
+    B.evo = E.evo_type[b];
+    B.evo_off = E.evo_off[b];
+    B.evo_len = E.evo_len[b];
+    B.left_index = E.left_index[b];
+    lds.m[lane] = B.valid ? B.m : 0.;
+    lds.mg[lane] = B.valid ? B.mg : 0.;
+    __syncthreads();
+    // Jacobi prefix constants (whfast.rs:574-576, 642-651)
+    double mu = B.hmg, eta = B.hm;
+    int rank = 0, my_rank = 0;
+    for (int j = 0; j < B.n; ++j) {
+        if (j == B.host) continue;
+        mu = mu + lds.mg[B.gbase + j];
+        eta = eta + lds.m[B.gbase + j];
+        if (j == k) { B.jac_mu = mu; B.jac_eta = eta; my_rank = rank; }
+        ++rank;
+    }
+    B.jac_rank = my_rank;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) { B.ax[q] = 0.; B.av[q] = 0.; B.aacc[q] = 0.; B.comx[q] = 0.; B.comv[q] = 0.; }
+    B.com_m = 0.;
+}
+
+__device__ inline void store_lane(const DevEnsemble &E, const Lane &B) {
+    if (!B.valid) return;
+    const int b = B.body;
+    const int nb = E.n_bodies;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        E.x[q * nb + b] = B.x[q];
+        E.v[q * nb + b] = B.v[q];
+        E.L[q * nb + b] = B.L[q];
+        E.spin[q * nb + b] = B.s[q];
+        E.err_v[q * nb + b] = B.ev[q];
+        E.err_L[q * nb + b] = B.eL[q];
+        E.acc[q * nb + b] = B.ag[q];
+    }
+    E.radius[b] = B.R;
+    E.rg2[b] = B.rg2;
+    E.inertia[b] = B.I;
+    E.left_index[b] = B.left_index;
+    if (B.k == 0) {
+        const int s = B.sys;
+        E.t[s] = B.t;
+        E.last_hist[s] = B.last_hist;
+        E.iteration[s] = B.iteration;
+        E.status[s] = B.status;
+        E.timestep_warning[s] = B.tw;
+    }
+}
+
+constexpr uint32_t FROZEN = POS_STATUS_COMPLETED | POS_STATUS_FATAL;
+
+// K steps of every system of every wave; the grid may be smaller than the
+// number of waves, in which case each workgroup walks the table with a
+// grid stride (persistent launch).
+__global__ __launch_bounds__(64) void whfast_steps_kernel(DevEnsemble E, int n_steps) {
+    __shared__ WaveLds lds;
+    for (int wave = blockIdx.x; wave < E.n_waves; wave += gridDim.x) {
+        Lane B;
+        load_lane(E, lds, B, wave);
+        for (int step = 0; step < n_steps; ++step) {
+            if (B.status & FROZEN) break;   // uniform within the group
+            iterate(E, lds, B);
+        }
+        store_lane(E, B);
+        __syncthreads();
+    }
+}
+
+// initialize_physical_values (whfast.rs:226-233) for systems at t == 0:
+// evolution at t, then spin = L / I.
+__global__ __launch_bounds__(64) void init_kernel(DevEnsemble E) {
+    __shared__ WaveLds lds;
+    const int wave = blockIdx.x;
+    if (wave >= E.n_waves) return;
+    Lane B;
+    load_lane(E, lds, B, wave);
+    if (B.t == 0.) {
+        if (B.evolution) evolve_body(E, B, B.t);
+        own_spin(B);
+        merge_status(lds, B);
+        store_lane(E, B);
+    }
+}
+
+// Operator-level evaluations (no state change).
+__global__ __launch_bounds__(64) void gravity_kernel(DevEnsemble E, double *out_acc, uint32_t *out_status) {
+    __shared__ WaveLds lds;
+    const int wave = blockIdx.x;
+    if (wave >= E.n_waves) return;
+    Lane B;
+    load_lane(E, lds, B, wave);
+    gravity(E, lds, B);
+    merge_status(lds, B);
+    const int nb = E.n_bodies;
+    if (B.valid) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) out_acc[q * nb + B.body] = B.ag[q];
+        if (B.k == 0) out_status[B.sys] = B.status;
+    }
+}
+
+__global__ __launch_bounds__(64) void effects_kernel(DevEnsemble E, double *out_acc, double *out_dLdt) {
+    __shared__ WaveLds lds;
+    const int wave = blockIdx.x;
+    if (wave >= E.n_waves) return;
+    Lane B;
+    load_lane(E, lds, B, wave);
+    double a[3], dL[3];
+    additional_effects(E, lds, B, false, a, dL);
+    const int nb = E.n_bodies;
+    if (B.valid) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) { out_acc[q * nb + B.body] = a[q]; out_dLdt[q * nb + B.body] = dL[q]; }
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(POS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T> struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct pos_ens {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    pos_options opts{};
+    int n_systems = 0, n_bodies = 0;
+    // device order <-> input order
+    std::vector<int> sys_perm;          // device system d -> input system
+    std::vector<int> body_perm;         // device body db -> input body
+    std::vector<int> body_offset_dev;   // device CSR
+    std::vector<int> body_offset_in;    // input CSR
+    std::vector<int> n_per_sys_dev;
+    // launch tables per mapping
+    std::vector<int> wave_first, wave_count, wave_group;
+    int *d_wave_first = nullptr, *d_wave_count = nullptr, *d_wave_group = nullptr;   // capacity n_systems
+    double *d_op_a = nullptr, *d_op_l = nullptr;   // operator outputs, [3][n_bodies]
+    uint32_t *d_op_st = nullptr;                    // operator status, [n_systems]
+    int steps_per_launch = 1;
+    int grid = 0;
+    // device allocations
+    std::vector<void *> allocs;
+    DevEnsemble E{};
+    // graph of graph_launches chunk launches
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_launches = 0;
+};
+
+namespace {
+
+template <class T> int dalloc(pos_ens *h, T **p, size_t n) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return fail(POS_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    h->allocs.push_back(q);
+    *p = (T *)q;
+    return POS_OK;
+}
+
+template <class T> int upload(pos_ens *h, T *dst, const std::vector<T> &src) {
+    if (src.empty()) return POS_OK;
+    HIP_TRY(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    return POS_OK;
+}
+
+template <class T> int download(pos_ens *h, std::vector<T> &dst, const T *src, size_t n) {
+    dst.resize(n);
+    if (n == 0) return POS_OK;
+    HIP_TRY(hipMemcpyAsync(dst.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+    return POS_OK;
+}
+
+
+int next_pow2(int n) {
+    int g = 1;
+    while (g < n) g <<= 1;
+    return g;
+}
+
+// Build the wave table for the current mapping; systems are in device order
+// (sorted by body count).
+int build_launch(pos_ens *h, const pos_options &o) {
+    h->wave_first.clear();
+    h->wave_count.clear();
+    h->wave_group.clear();
+    const int ns = h->n_systems;
+    if (o.mapping == POS_MAPPING_PLAIN) {
+        for (int s = 0; s < ns; ++s) {
+            h->wave_first.push_back(s);
+            h->wave_count.push_back(1);
+            h->wave_group.push_back(64);
+        }
+        h->steps_per_launch = 1;
+    } else {
+        int s = 0;
+        while (s < ns) {
+            const int n = h->n_per_sys_dev[s];
+            int G = o.group_size > 0 ? std::max(o.group_size, n) : next_pow2(n);
+            if (G > 64) G = 64;
+            const int per_wave = 64 / G;
+            // systems with the same group size share waves
+            int e = s;
+            while (e < ns) {
+                const int ne = h->n_per_sys_dev[e];
+                const int Ge = o.group_size > 0 ? std::max(o.group_size, ne) : next_pow2(ne);
+                if (std::min(Ge, 64) != G) break;
+                ++e;
+            }
+            for (int f = s; f < e; f += per_wave) {
+                h->wave_first.push_back(f);
+                h->wave_count.push_back(std::min(per_wave, e - f));
+                h->wave_group.push_back(G);
+            }
+            s = e;
+        }
+        h->steps_per_launch = o.steps_per_launch > 0 ? o.steps_per_launch : 64;
+    }
+    const int nw = (int)h->wave_first.size();
+    int rc;
+    if (!h->d_wave_first) {
+        if ((rc = dalloc(h, &h->d_wave_first, ns)) || (rc = dalloc(h, &h->d_wave_count, ns)) ||
+            (rc = dalloc(h, &h->d_wave_group, ns)))
+            return rc;
+    }
+    if ((rc = upload(h, h->d_wave_first, h->wave_first)) || (rc = upload(h, h->d_wave_count, h->wave_count)) ||
+        (rc = upload(h, h->d_wave_group, h->wave_group)))
+        return rc;
+    h->E.wave_first = h->d_wave_first;
+    h->E.wave_count = h->d_wave_count;
+    h->E.wave_group = h->d_wave_group;
+    h->E.n_waves = nw;
+    h->grid = o.reserved[0] > 0 ? std::min(o.reserved[0], nw) : nw;   // reserved[0]: persistent grid size
+    h->opts = o;
+    h->opts.steps_per_launch = h->steps_per_launch;
+    if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    h->graph_launches = 0;
+    return POS_OK;
+}
+
+int validate(const pos_ensemble_desc *d) {
+    if (!d) return fail(POS_E_INVALID, "desc is NULL");
+    if (d->n_systems <= 0) return fail(POS_E_INVALID, "n_systems must be positive");
+    const void *required[] = {d->body_offset, d->coordinates, d->effects, d->gr_implementation, d->time_step,
+                              d->time_limit, d->mass, d->radius, d->radius_of_gyration_2, d->position,
+                              d->velocity, d->angular_momentum, d->roles, d->tides_dissipation_factor,
+                              d->tides_dissipation_factor_scale, d->tides_love_number, d->flattening_love_number};
+    for (const void *p : required)
+        if (!p) return fail(POS_E_INVALID, "a required desc array is NULL");
+    if (d->body_offset[0] != 0) return fail(POS_E_INVALID, "body_offset[0] must be 0");
+    for (int s = 0; s < d->n_systems; ++s) {
+        const int b0 = d->body_offset[s], n = d->body_offset[s + 1] - b0;
+        const std::string where = "system " + std::to_string(s) + ": ";
+        if (n < 2 || n > POS_MAX_BODIES)
+            return fail(POS_E_LIMIT, where + "body count " + std::to_string(n) + " outside [2, " + std::to_string(POS_MAX_BODIES) + "]");
+        const uint32_t eff = d->effects[s];
+        if (eff & POS_EFFECT_DISK) return fail(POS_E_UNSUPPORTED, where + "disk effect is not supported");
+        if (d->coordinates[s] < 0 || d->coordinates[s] > 2) return fail(POS_E_INVALID, where + "bad coordinates");
+        if ((eff & POS_EFFECT_GENERAL_RELATIVITY) && d->gr_implementation[s] != POS_GR_KIDDER1995)
+            return fail(POS_E_UNSUPPORTED, where + "only Kidder1995 general relativity is supported");
+        if (!(d->time_step[s] > 0.)) return fail(POS_E_INVALID, where + "time_step must be positive");
+        int mm = -1;
+        double mmax = 0.;
+        for (int i = 0; i < n; ++i)
+            if (d->mass[b0 + i] > mmax) { mmax = d->mass[b0 + i]; mm = i; }
+        if (mm < 0) return fail(POS_E_INVALID, where + "no body has a positive mass");
+        const uint32_t hr = d->roles[b0 + mm];
+        if ((eff & POS_EFFECT_TIDES) && !(hr & POS_ROLE_TIDES_CENTRAL))
+            return fail(POS_E_UNSUPPORTED, where + "the tidal host must be the most massive body");
+        if ((eff & POS_EFFECT_ROTATIONAL_FLATTENING) && !(hr & POS_ROLE_FLAT_CENTRAL))
+            return fail(POS_E_UNSUPPORTED, where + "the flattening host must be the most massive body");
+        if ((eff & POS_EFFECT_GENERAL_RELATIVITY) && !(hr & POS_ROLE_GR_CENTRAL))
+            return fail(POS_E_UNSUPPORTED, where + "the GR host must be the most massive body");
+        for (int i = 0; i < n; ++i) {
+            if (i == mm) continue;
+            const uint32_t r = d->roles[b0 + i];
+            if (r & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_FLAT_CENTRAL | POS_ROLE_GR_CENTRAL))
+                return fail(POS_E_UNSUPPORTED, where + "only one central body per system");
+            if ((eff & POS_EFFECT_TIDES) && !(r & POS_ROLE_TIDES_ORBITING))
+                return fail(POS_E_UNSUPPORTED, where + "with tides on every other body must be a tidal orbiting body");
+            if ((eff & POS_EFFECT_ROTATIONAL_FLATTENING) && !(r & POS_ROLE_FLAT_ORBITING))
+                return fail(POS_E_UNSUPPORTED, where + "with flattening on every other body must be an orbiting body");
+            if ((eff & POS_EFFECT_GENERAL_RELATIVITY) && !(r & POS_ROLE_GR_ORBITING))
+                return fail(POS_E_UNSUPPORTED, where + "with GR on every other body must be an orbiting body");
+        }
+        const double last = d->last_historic_snapshot_time ? d->last_historic_snapshot_time[s] : -1.;
+        if (last >= 0. && d->heliocentric_velocity) {
+            const double *hv = d->heliocentric_velocity + 3 * (b0 + mm);
+            if (hv[0] != 0. || hv[1] != 0. || hv[2] != 0.)
+                return fail(POS_E_UNSUPPORTED, where + "a resumed system must have a zero host heliocentric velocity");
+        }
+        if (d->evolution_type) {
+            for (int i = 0; i < n; ++i) {
+                const int et = d->evolution_type[b0 + i];
+                if (et < POS_EVO_NON_EVOLVING || et > POS_EVO_LECONTE_CHABRIER2013)
+                    return fail(POS_E_UNSUPPORTED, where + "evolving-dissipation models are not supported");
+                if (et != POS_EVO_NON_EVOLVING &&
+                    (!d->evolver_offset || d->evolver_offset[b0 + i + 1] - d->evolver_offset[b0 + i] <= 0))
+                    return fail(POS_E_INVALID, where + "evolving body without a table");
+                if (et != POS_EVO_NON_EVOLVING && d->evolver_left_index && d->evolver_left_index[b0 + i] < 0)
+                    return fail(POS_E_INVALID, where + "negative evolver left_index");
+            }
+        }
+    }
+    return POS_OK;
+}
+
+double host_roche(double ma, double mb, double ra, double rb) {
+    // universe.rs:186-193 (setup only, evaluated with the host libm)
+    if (ma > mb) return (rb / 0.462) * std::pow(ma / mb, 1. / 3.);
+    return (ra / 0.462) * std::pow(mb / ma, 1. / 3.);
+}
+
+constexpr int GRAPH_LAUNCHES = 4;   // kernel launches per captured graph
+
+int launch_steps(pos_ens *h, int n_steps) {
+    hipLaunchKernelGGL(whfast_steps_kernel, dim3(h->grid), dim3(64), 0, h->stream, h->E, n_steps);
+    HIP_TRY(hipGetLastError());
+    return POS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t pos_abi_version(void) { return POS_ABI_VERSION; }
+
+const char *pos_last_error(void) { return g_last_error.c_str(); }
+
+int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens **out) {
+    if (!out) return fail(POS_E_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate(d);
+    if (rc) return rc;
+    pos_options o{};
+    if (opts) o = *opts;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(POS_E_NO_DEVICE, "no HIP device");
+    if (o.device < 0 || o.device >= ndev) return fail(POS_E_INVALID, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(o.device));
+
+    pos_ens *h = new pos_ens();
+    h->device = o.device;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(POS_E_HIP, "hipStreamCreate failed");
+    }
+    const int ns = d->n_systems;
+    const int nb = d->body_offset[ns];
+    h->n_systems = ns;
+    h->n_bodies = nb;
+    h->body_offset_in.assign(d->body_offset, d->body_offset + ns + 1);
+
+    // device order: by body count, then input order
+    h->sys_perm.resize(ns);
+    std::iota(h->sys_perm.begin(), h->sys_perm.end(), 0);
+    std::stable_sort(h->sys_perm.begin(), h->sys_perm.end(), [&](int a, int b) {
+        return (d->body_offset[a + 1] - d->body_offset[a]) < (d->body_offset[b + 1] - d->body_offset[b]);
+    });
+    h->body_offset_dev.assign(ns + 1, 0);
+    h->n_per_sys_dev.resize(ns);
+    h->body_perm.resize(nb);
+    for (int ds = 0; ds < ns; ++ds) {
+        const int s = h->sys_perm[ds];
+        const int b0 = d->body_offset[s], n = d->body_offset[s + 1] - b0;
+        h->n_per_sys_dev[ds] = n;
+        h->body_offset_dev[ds + 1] = h->body_offset_dev[ds] + n;
+        for (int i = 0; i < n; ++i) h->body_perm[h->body_offset_dev[ds] + i] = b0 + i;
+    }
+
+    // ---- host-side arrays in device order ----
+    std::vector<int> host_idx(ns), coord(ns), tw(ns, 0);
+    std::vector<uint32_t> eff(ns), status(ns, 0);
+    std::vector<double> dt(ns), tl(ns), t(ns), lh(ns), hp(ns);
+    std::vector<long long> iter(ns);
+    for (int ds = 0; ds < ns; ++ds) {
+        const int s = h->sys_perm[ds];
+        const int b0 = d->body_offset[s], n = d->body_offset[s + 1] - b0;
+        int mm = 0;
+        double mmax = 0.;
+        for (int i = 0; i < n; ++i)
+            if (d->mass[b0 + i] > mmax) { mmax = d->mass[b0 + i]; mm = i; }
+        host_idx[ds] = mm;
+        coord[ds] = d->coordinates[s];
+        uint32_t e = d->effects[s];
+        eff[ds] = e;
+        dt[ds] = d->time_step[s];
+        tl[ds] = d->time_limit[s];
+        t[ds] = d->current_time ? d->current_time[s] : 0.;
+        lh[ds] = d->last_historic_snapshot_time ? d->last_historic_snapshot_time[s] : -1.;
+        hp[ds] = d->historic_snapshot_period ? d->historic_snapshot_period[s] : std::numeric_limits<double>::infinity();
+        iter[ds] = d->current_iteration ? (long long)d->current_iteration[s] : 0;
+    }
+    std::vector<double> x(3 * nb), v(3 * nb), L(3 * nb), sp(3 * nb), ev(3 * nb, 0.), eL(3 * nb, 0.), acc(3 * nb, 0.);
+    std::vector<double> m(nb), R(nb), rg2(nb), I(nb), sigma(nb), k2(nb), k2f(nb), wk(nb, 0.), ws(nb, 0.);
+    std::vector<uint32_t> roles(nb);
+    std::vector<int> evo_type(nb, POS_EVO_NON_EVOLVING), evo_off(nb, 0), evo_len(nb, 0), left(nb, 0);
+    std::vector<double> tab_t, tab_R, tab_rg2, tab_love;
+    for (int db = 0; db < nb; ++db) {
+        const int b = h->body_perm[db];
+        m[db] = d->mass[b];
+        R[db] = d->radius[b];
+        rg2[db] = d->radius_of_gyration_2[b];
+        I[db] = d->moment_of_inertia ? d->moment_of_inertia[b] : m[db] * rg2[db] * (R[db] * R[db]);
+        sigma[db] = d->tides_dissipation_factor_scale[b] * d->tides_dissipation_factor[b];
+        k2[db] = d->tides_love_number[b];
+        k2f[db] = d->flattening_love_number[b];
+        if (d->wind_k_factor) wk[db] = d->wind_k_factor[b];
+        if (d->wind_rotation_saturation) ws[db] = d->wind_rotation_saturation[b];
+        roles[db] = d->roles[b];
+        for (int q = 0; q < 3; ++q) {
+            x[q * nb + db] = d->position[3 * b + q];
+            v[q * nb + db] = d->velocity[3 * b + q];
+            L[q * nb + db] = d->angular_momentum[3 * b + q];
+            sp[q * nb + db] = d->spin ? d->spin[3 * b + q] : L[q * nb + db] / I[db];
+            if (d->velocity_error) ev[q * nb + db] = d->velocity_error[3 * b + q];
+            if (d->angular_momentum_error) eL[q * nb + db] = d->angular_momentum_error[3 * b + q];
+        }
+        if (d->evolution_type && d->evolution_type[b] != POS_EVO_NON_EVOLVING) {
+            const int o0 = d->evolver_offset[b], len = d->evolver_offset[b + 1] - o0;
+            evo_type[db] = d->evolution_type[b];
+            evo_off[db] = (int)tab_t.size();
+            evo_len[db] = len;
+            left[db] = d->evolver_left_index ? d->evolver_left_index[b] : 0;
+            tab_t.insert(tab_t.end(), d->evolver_time + o0, d->evolver_time + o0 + len);
+            tab_R.insert(tab_R.end(), d->evolver_radius + o0, d->evolver_radius + o0 + len);
+            tab_rg2.insert(tab_rg2.end(), d->evolver_radius_of_gyration_2 + o0, d->evolver_radius_of_gyration_2 + o0 + len);
+            tab_love.insert(tab_love.end(), d->evolver_love_number + o0, d->evolver_love_number + o0 + len);
+        }
+    }
+
+    // ---- device allocations ----
+    DevEnsemble &E = h->E;
+    E.n_systems = ns;
+    E.n_bodies = nb;
+    int *d_bo, *d_host, *d_coord, *d_tw, *d_evo_type, *d_evo_off, *d_evo_len, *d_left;
+    uint32_t *d_eff, *d_status, *d_roles;
+    double *d_dt, *d_tl, *d_t, *d_lh, *d_hp;
+    long long *d_iter;
+    double *d_x, *d_v, *d_L, *d_sp, *d_ev, *d_eL, *d_acc, *d_R, *d_rg2, *d_I, *d_m, *d_sigma, *d_k2, *d_k2f, *d_wk, *d_ws, *d_roche;
+    double *d_tt, *d_tR, *d_trg2, *d_tlove;
+    if ((rc = dalloc(h, &d_bo, ns + 1)) || (rc = dalloc(h, &d_host, ns)) || (rc = dalloc(h, &d_coord, ns)) ||
+        (rc = dalloc(h, &d_tw, ns)) || (rc = dalloc(h, &d_eff, ns)) || (rc = dalloc(h, &d_status, ns)) ||
+        (rc = dalloc(h, &d_dt, ns)) || (rc = dalloc(h, &d_tl, ns)) || (rc = dalloc(h, &d_t, ns)) ||
+        (rc = dalloc(h, &d_lh, ns)) || (rc = dalloc(h, &d_hp, ns)) || (rc = dalloc(h, &d_iter, ns)) ||
+        (rc = dalloc(h, &d_x, 3 * nb)) || (rc = dalloc(h, &d_v, 3 * nb)) || (rc = dalloc(h, &d_L, 3 * nb)) ||
+        (rc = dalloc(h, &d_sp, 3 * nb)) || (rc = dalloc(h, &d_ev, 3 * nb)) || (rc = dalloc(h, &d_eL, 3 * nb)) ||
+        (rc = dalloc(h, &d_acc, 3 * nb)) || (rc = dalloc(h, &d_R, nb)) || (rc = dalloc(h, &d_rg2, nb)) ||
+        (rc = dalloc(h, &d_I, nb)) || (rc = dalloc(h, &d_m, nb)) || (rc = dalloc(h, &d_sigma, nb)) ||
+        (rc = dalloc(h, &d_k2, nb)) || (rc = dalloc(h, &d_k2f, nb)) || (rc = dalloc(h, &d_wk, nb)) ||
+        (rc = dalloc(h, &d_ws, nb)) || (rc = dalloc(h, &d_roles, nb)) || (rc = dalloc(h, &d_roche, (size_t)nb * POS_MAX_BODIES)) ||
+        (rc = dalloc(h, &d_evo_type, nb)) || (rc = dalloc(h, &d_evo_off, nb)) || (rc = dalloc(h, &d_evo_len, nb)) ||
+        (rc = dalloc(h, &d_left, nb)) || (rc = dalloc(h, &d_tt, tab_t.size())) || (rc = dalloc(h, &d_tR, tab_t.size())) ||
+        (rc = dalloc(h, &d_trg2, tab_t.size())) || (rc = dalloc(h, &d_tlove, tab_t.size()))) {
+        pos_destroy(h);
+        return rc;
+    }
+    E.body_offset = d_bo; E.host = d_host; E.coord = d_coord; E.effects = d_eff;
+    E.dt = d_dt; E.time_limit = d_tl; E.hist_period = d_hp; E.t = d_t; E.last_hist = d_lh;
+    E.iteration = d_iter; E.status = d_status; E.timestep_warning = d_tw;
+    E.x = d_x; E.v = d_v; E.L = d_L; E.spin = d_sp; E.err_v = d_ev; E.err_L = d_eL; E.acc = d_acc;
+    E.radius = d_R; E.rg2 = d_rg2; E.inertia = d_I;
+    E.mass = d_m; E.sigma = d_sigma; E.k2 = d_k2; E.k2f = d_k2f; E.wind_k = d_wk; E.wind_sat = d_ws; E.roles = d_roles;
+    E.roche = d_roche;
+    E.evo_type = d_evo_type; E.evo_off = d_evo_off; E.evo_len = d_evo_len; E.left_index = d_left;
+    E.tab_time = d_tt; E.tab_radius = d_tR; E.tab_rg2 = d_trg2; E.tab_love = d_tlove;
+
+    if ((rc = upload(h, d_bo, h->body_offset_dev)) || (rc = upload(h, d_host, host_idx)) || (rc = upload(h, d_coord, coord)) ||
+        (rc = upload(h, d_tw, tw)) || (rc = upload(h, d_eff, eff)) || (rc = upload(h, d_status, status)) ||
+        (rc = upload(h, d_dt, dt)) || (rc = upload(h, d_tl, tl)) || (rc = upload(h, d_t, t)) || (rc = upload(h, d_lh, lh)) ||
+        (rc = upload(h, d_hp, hp)) || (rc = upload(h, d_iter, iter)) || (rc = upload(h, d_x, x)) || (rc = upload(h, d_v, v)) ||
+        (rc = upload(h, d_L, L)) || (rc = upload(h, d_sp, sp)) || (rc = upload(h, d_ev, ev)) || (rc = upload(h, d_eL, eL)) ||
+        (rc = upload(h, d_acc, acc)) || (rc = upload(h, d_R, R)) || (rc = upload(h, d_rg2, rg2)) || (rc = upload(h, d_I, I)) ||
+        (rc = upload(h, d_m, m)) || (rc = upload(h, d_sigma, sigma)) || (rc = upload(h, d_k2, k2)) || (rc = upload(h, d_k2f, k2f)) ||
+        (rc = upload(h, d_wk, wk)) || (rc = upload(h, d_ws, ws)) || (rc = upload(h, d_roles, roles)) ||
+        (rc = upload(h, d_evo_type, evo_type)) || (rc = upload(h, d_evo_off, evo_off)) || (rc = upload(h, d_evo_len, evo_len)) ||
+        (rc = upload(h, d_left, left)) || (rc = upload(h, d_tt, tab_t)) || (rc = upload(h, d_tR, tab_R)) ||
+        (rc = upload(h, d_trg2, tab_rg2)) || (rc = upload(h, d_tlove, tab_love))) {
+        pos_destroy(h);
+        return rc;
+    }
+
+    // initialize_physical_values on the device, one system per wave
+    {
+        pos_options init_o{};
+        init_o.mapping = POS_MAPPING_PLAIN;
+        if ((rc = build_launch(h, init_o))) { pos_destroy(h); return rc; }
+        hipLaunchKernelGGL(init_kernel, dim3(h->E.n_waves), dim3(64), 0, h->stream, h->E);
+        if (hipGetLastError() != hipSuccess) { pos_destroy(h); return fail(POS_E_HIP, "init kernel launch failed"); }
+    }
+    // Roche radii from the (possibly evolved) radii (universe.rs:177-196)
+    {
+        std::vector<double> Rnow;
+        if ((rc = download(h, Rnow, d_R, nb))) { pos_destroy(h); return rc; }
+        if (hipStreamSynchronize(h->stream) != hipSuccess) { pos_destroy(h); return fail(POS_E_HIP, "synchronize failed"); }
+        std::vector<double> roche((size_t)nb * POS_MAX_BODIES, 0.);
+        for (int ds = 0; ds < ns; ++ds) {
+            const int b0 = h->body_offset_dev[ds], n = h->n_per_sys_dev[ds];
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j)
+                    if (i != j)
+                        roche[(size_t)(b0 + i) * POS_MAX_BODIES + j] = host_roche(m[b0 + i], m[b0 + j], Rnow[b0 + i], Rnow[b0 + j]);
+        }
+        if ((rc = upload(h, d_roche, roche))) { pos_destroy(h); return rc; }
+    }
+    if ((rc = build_launch(h, o))) { pos_destroy(h); return rc; }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) { pos_destroy(h); return fail(POS_E_HIP, "synchronize failed"); }
+    *out = h;
+    return POS_OK;
+}
+
+int32_t pos_step(pos_ens *h, int64_t n_steps) {
+    if (!h) return fail(POS_E_INVALID, "handle is NULL");
+    if (n_steps < 0) return fail(POS_E_INVALID, "n_steps must be >= 0");
+    HIP_TRY(hipSetDevice(h->device));
+    const int K = h->steps_per_launch;
+    int64_t full = n_steps / K;
+    const int rest = (int)(n_steps - full * K);
+    if (h->opts.use_graph && full >= GRAPH_LAUNCHES) {
+        // GRAPH_LAUNCHES chunk launches captured once per launch table, replayed
+        if (!h->graph_exec) {
+            HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < GRAPH_LAUNCHES; ++i)
+                hipLaunchKernelGGL(whfast_steps_kernel, dim3(h->grid), dim3(64), 0, h->stream, h->E, K);
+            HIP_TRY(hipStreamEndCapture(h->stream, &h->graph));
+            HIP_TRY(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+            h->graph_launches = GRAPH_LAUNCHES;
+        }
+        while (full >= GRAPH_LAUNCHES) {
+            HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+            full -= GRAPH_LAUNCHES;
+        }
+    }
+    int rc;
+    for (int64_t i = 0; i < full; ++i)
+        if ((rc = launch_steps(h, K))) return rc;
+    if (rest > 0 && (rc = launch_steps(h, rest))) return rc;
+    return POS_OK;
+}
+
+int32_t pos_synchronize(pos_ens *h) {
+    if (!h) return fail(POS_E_INVALID, "handle is NULL");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return POS_OK;
+}
+
+int32_t pos_get_state(pos_ens *h, pos_state *o) {
+    if (!h || !o) return fail(POS_E_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(h->device));
+    const DevEnsemble &E = h->E;
+    const int ns = h->n_systems, nb = h->n_bodies;
+    std::vector<double> buf;
+    int rc;
+    auto vec3 = [&](double *dst, double *src) -> int {
+        if (!dst) return POS_OK;
+        int r = download(h, buf, src, 3 * (size_t)nb);
+        if (r) return r;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (int db = 0; db < nb; ++db) {
+            const int b = h->body_perm[db];
+            for (int q = 0; q < 3; ++q) dst[3 * b + q] = buf[(size_t)q * nb + db];
+        }
+        return POS_OK;
+    };
+    auto vec1 = [&](double *dst, double *src) -> int {
+        if (!dst) return POS_OK;
+        int r = download(h, buf, src, nb);
+        if (r) return r;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (int db = 0; db < nb; ++db) dst[h->body_perm[db]] = buf[db];
+        return POS_OK;
+    };
+    if ((rc = vec3(o->position, E.x)) || (rc = vec3(o->velocity, E.v)) || (rc = vec3(o->acceleration, E.acc)) ||
+        (rc = vec3(o->angular_momentum, E.L)) || (rc = vec3(o->spin, E.spin)) || (rc = vec3(o->velocity_error, E.err_v)) ||
+        (rc = vec3(o->angular_momentum_error, E.err_L)) || (rc = vec1(o->radius, E.radius)) ||
+        (rc = vec1(o->radius_of_gyration_2, E.rg2)) || (rc = vec1(o->moment_of_inertia, E.inertia)))
+        return rc;
+    auto sys_d = [&](double *dst, double *src) -> int {
+        if (!dst) return POS_OK;
+        int r = download(h, buf, src, ns);
+        if (r) return r;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (int ds = 0; ds < ns; ++ds) dst[h->sys_perm[ds]] = buf[ds];
+        return POS_OK;
+    };
+    if ((rc = sys_d(o->current_time, E.t)) || (rc = sys_d(o->last_historic_snapshot_time, E.last_hist))) return rc;
+    if (o->current_iteration) {
+        std::vector<long long> it;
+        if ((rc = download(h, it, E.iteration, ns))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (int ds = 0; ds < ns; ++ds) o->current_iteration[h->sys_perm[ds]] = it[ds];
+    }
+    if (o->status) {
+        std::vector<uint32_t> st;
+        if ((rc = download(h, st, E.status, ns))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (int ds = 0; ds < ns; ++ds) o->status[h->sys_perm[ds]] = st[ds];
+    }
+    return POS_OK;
+}
+
+int32_t pos_set_state(pos_ens *h, const pos_state *in) {
+    if (!h || !in) return fail(POS_E_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(h->device));
+    const DevEnsemble &E = h->E;
+    const int ns = h->n_systems, nb = h->n_bodies;
+    std::vector<double> buf;
+    int rc;
+    auto vec3 = [&](const double *src, double *dst) -> int {
+        if (!src) return POS_OK;
+        buf.assign(3 * (size_t)nb, 0.);
+        for (int db = 0; db < nb; ++db) {
+            const int b = h->body_perm[db];
+            for (int q = 0; q < 3; ++q) buf[(size_t)q * nb + db] = src[3 * b + q];
+        }
+        int r = upload(h, dst, buf);
+        if (r) return r;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return POS_OK;
+    };
+    auto vec1 = [&](const double *src, double *dst) -> int {
+        if (!src) return POS_OK;
+        buf.assign(nb, 0.);
+        for (int db = 0; db < nb; ++db) buf[db] = src[h->body_perm[db]];
+        int r = upload(h, dst, buf);
+        if (r) return r;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return POS_OK;
+    };
+    if ((rc = vec3(in->position, E.x)) || (rc = vec3(in->velocity, E.v)) || (rc = vec3(in->acceleration, E.acc)) ||
+        (rc = vec3(in->angular_momentum, E.L)) || (rc = vec3(in->spin, E.spin)) || (rc = vec3(in->velocity_error, E.err_v)) ||
+        (rc = vec3(in->angular_momentum_error, E.err_L)) || (rc = vec1(in->radius, E.radius)) ||
+        (rc = vec1(in->radius_of_gyration_2, E.rg2)) || (rc = vec1(in->moment_of_inertia, E.inertia)))
+        return rc;
+    auto sys_d = [&](const double *src, double *dst) -> int {
+        if (!src) return POS_OK;
+        buf.assign(ns, 0.);
+        for (int ds = 0; ds < ns; ++ds) buf[ds] = src[h->sys_perm[ds]];
+        int r = upload(h, dst, buf);
+        if (r) return r;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return POS_OK;
+    };
+    if ((rc = sys_d(in->current_time, E.t)) || (rc = sys_d(in->last_historic_snapshot_time, E.last_hist))) return rc;
+    if (in->current_iteration) {
+        std::vector<long long> it(ns);
+        for (int ds = 0; ds < ns; ++ds) it[ds] = in->current_iteration[h->sys_perm[ds]];
+        if ((rc = upload(h, E.iteration, it))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    if (in->status) {
+        std::vector<uint32_t> st(ns);
+        for (int ds = 0; ds < ns; ++ds) st[ds] = in->status[h->sys_perm[ds]];
+        if ((rc = upload(h, E.status, st))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    return POS_OK;
+}
+
+int32_t pos_set_time_limit(pos_ens *h, double time_limit) {
+    if (!h) return fail(POS_E_INVALID, "handle is NULL");
+    if (!(time_limit > 0.)) return POS_OK;
+    HIP_TRY(hipSetDevice(h->device));
+    std::vector<double> tl(h->n_systems, time_limit);
+    int rc = upload(h, const_cast<double *>(h->E.time_limit), tl);
+    if (rc) return rc;
+    // a system re-opens if the new limit lies beyond t + dt
+    std::vector<uint32_t> st;
+    std::vector<double> t, dt;
+    if ((rc = download(h, st, h->E.status, h->n_systems)) || (rc = download(h, t, h->E.t, h->n_systems)) ||
+        (rc = download(h, dt, h->E.dt, h->n_systems)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int s = 0; s < h->n_systems; ++s)
+        if (!(t[s] + dt[s] > time_limit)) st[s] &= ~(uint32_t)POS_STATUS_COMPLETED;
+    if ((rc = upload(h, h->E.status, st))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return POS_OK;
+}
+
+int32_t pos_set_historic_snapshot_period(pos_ens *h, double period) {
+    if (!h) return fail(POS_E_INVALID, "handle is NULL");
+    if (!(period > 0.)) return POS_OK;
+    HIP_TRY(hipSetDevice(h->device));
+    std::vector<double> hp(h->n_systems, period);
+    int rc = upload(h, const_cast<double *>(h->E.hist_period), hp);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return POS_OK;
+}
+
+int32_t pos_get_counts(const pos_ens *h, int32_t *n_systems, int32_t *n_bodies) {
+    if (!h) return fail(POS_E_INVALID, "handle is NULL");
+    if (n_systems) *n_systems = h->n_systems;
+    if (n_bodies) *n_bodies = h->n_bodies;
+    return POS_OK;
+}
+
+int32_t pos_get_stream(const pos_ens *h, void **stream) {
+    if (!h || !stream) return fail(POS_E_INVALID, "NULL argument");
+    *stream = (void *)h->stream;
+    return POS_OK;
+}
+
+int32_t pos_get_options(const pos_ens *h, pos_options *out) {
+    if (!h || !out) return fail(POS_E_INVALID, "NULL argument");
+    *out = h->opts;
+    return POS_OK;
+}
+
+int32_t pos_gravity_acceleration(pos_ens *h, double *out_acc, uint32_t *out_status) {
+    if (!h || !out_acc) return fail(POS_E_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(h->device));
+    const int ns = h->n_systems, nb = h->n_bodies;
+    pos_options saved = h->opts;
+    int rc;
+    pos_options plain{};
+    plain.mapping = POS_MAPPING_PLAIN;
+    if ((rc = build_launch(h, plain))) return rc;
+    if (!h->d_op_a && ((rc = dalloc(h, &h->d_op_a, 3 * (size_t)nb)) || (rc = dalloc(h, &h->d_op_l, 3 * (size_t)nb)) ||
+                       (rc = dalloc(h, &h->d_op_st, ns))))
+        return rc;
+    double *d_out = h->d_op_a;
+    uint32_t *d_st = h->d_op_st;
+    hipLaunchKernelGGL(gravity_kernel, dim3(h->E.n_waves), dim3(64), 0, h->stream, h->E, d_out, d_st);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> a;
+    std::vector<uint32_t> st;
+    if ((rc = download(h, a, d_out, 3 * (size_t)nb)) || (rc = download(h, st, d_st, ns))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int db = 0; db < nb; ++db)
+        for (int q = 0; q < 3; ++q) out_acc[3 * h->body_perm[db] + q] = a[(size_t)q * nb + db];
+    if (out_status)
+        for (int ds = 0; ds < ns; ++ds) out_status[h->sys_perm[ds]] |= st[ds];
+    return build_launch(h, saved);
+}
+
+int32_t pos_additional_effects(pos_ens *h, double *out_acc, double *out_dLdt) {
+    if (!h || !out_acc || !out_dLdt) return fail(POS_E_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(h->device));
+    const int nb = h->n_bodies;
+    pos_options saved = h->opts;
+    int rc;
+    pos_options plain{};
+    plain.mapping = POS_MAPPING_PLAIN;
+    if ((rc = build_launch(h, plain))) return rc;
+    if (!h->d_op_a && ((rc = dalloc(h, &h->d_op_a, 3 * (size_t)nb)) || (rc = dalloc(h, &h->d_op_l, 3 * (size_t)nb)) ||
+                       (rc = dalloc(h, &h->d_op_st, h->n_systems))))
+        return rc;
+    double *d_a = h->d_op_a, *d_l = h->d_op_l;
+    hipLaunchKernelGGL(effects_kernel, dim3(h->E.n_waves), dim3(64), 0, h->stream, h->E, d_a, d_l);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> a, l;
+    if ((rc = download(h, a, d_a, 3 * (size_t)nb)) || (rc = download(h, l, d_l, 3 * (size_t)nb))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int db = 0; db < nb; ++db)
+        for (int q = 0; q < 3; ++q) {
+            out_acc[3 * h->body_perm[db] + q] = a[(size_t)q * nb + db];
+            out_dLdt[3 * h->body_perm[db] + q] = l[(size_t)q * nb + db];
+        }
+    return build_launch(h, saved);
+}
+
+int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
+    if (!h) return fail(POS_E_INVALID, "handle is NULL");
+    if (probe_steps <= 0) probe_steps = 32;
+    HIP_TRY(hipSetDevice(h->device));
+    // snapshot the mutable state so that probing leaves it untouched
+    const int ns = h->n_systems, nb = h->n_bodies;
+    std::vector<double> sx(3 * (size_t)nb), sv(3 * (size_t)nb), sL(3 * (size_t)nb), ss(3 * (size_t)nb), sev(3 * (size_t)nb),
+        seL(3 * (size_t)nb), sacc(3 * (size_t)nb), sR(nb), srg2(nb), sI(nb), st(ns), slh(ns);
+    std::vector<long long> sit(ns);
+    std::vector<uint32_t> sst(ns);
+    std::vector<int> stw(ns), sleft(nb);
+    int rc;
+    const DevEnsemble &E = h->E;
+    if ((rc = download(h, sx, E.x, 3 * (size_t)nb)) || (rc = download(h, sv, E.v, 3 * (size_t)nb)) ||
+        (rc = download(h, sL, E.L, 3 * (size_t)nb)) || (rc = download(h, ss, E.spin, 3 * (size_t)nb)) ||
+        (rc = download(h, sev, E.err_v, 3 * (size_t)nb)) || (rc = download(h, seL, E.err_L, 3 * (size_t)nb)) ||
+        (rc = download(h, sacc, E.acc, 3 * (size_t)nb)) || (rc = download(h, sR, E.radius, nb)) ||
+        (rc = download(h, srg2, E.rg2, nb)) || (rc = download(h, sI, E.inertia, nb)) || (rc = download(h, st, E.t, ns)) ||
+        (rc = download(h, slh, E.last_hist, ns)) || (rc = download(h, sit, E.iteration, ns)) ||
+        (rc = download(h, sst, E.status, ns)) || (rc = download(h, stw, E.timestep_warning, ns)) ||
+        (rc = download(h, sleft, E.left_index, nb)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    auto restore = [&]() -> int {
+        int r;
+        if ((r = upload(h, E.x, sx)) || (r = upload(h, E.v, sv)) || (r = upload(h, E.L, sL)) || (r = upload(h, E.spin, ss)) ||
+            (r = upload(h, E.err_v, sev)) || (r = upload(h, E.err_L, seL)) || (r = upload(h, E.acc, sacc)) ||
+            (r = upload(h, E.radius, sR)) || (r = upload(h, E.rg2, srg2)) || (r = upload(h, E.inertia, sI)) ||
+            (r = upload(h, E.t, st)) || (r = upload(h, E.last_hist, slh)) || (r = upload(h, E.iteration, sit)) ||
+            (r = upload(h, E.status, sst)) || (r = upload(h, E.timestep_warning, stw)) || (r = upload(h, E.left_index, sleft)))
+            return r;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return POS_OK;
+    };
+    int max_n = 0;
+    for (int n : h->n_per_sys_dev) max_n = std::max(max_n, n);
+    std::vector<int> groups = {0};   // 0: next power of two per size class
+    for (int g = max_n; g <= 16; ++g)
+        if (64 % g != 0) groups.push_back(g);   // exact packing for sizes that do not divide 64
+    std::vector<int> ks = {16, 64};
+    const int keep_graph = h->opts.use_graph;
+    pos_options best = h->opts;
+    double best_t = std::numeric_limits<double>::infinity();
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int g : groups) {
+        for (int K : ks) {
+            pos_options o{};
+            o.device = h->device;
+            o.mapping = POS_MAPPING_PACKED;
+            o.group_size = g;
+            o.steps_per_launch = K;
+            o.use_graph = 0;
+            if ((rc = build_launch(h, o))) return rc;
+            const int steps = std::max(probe_steps, K);
+            if ((rc = pos_step(h, K))) return rc;   // warm-up
+            HIP_TRY(hipEventRecord(e0, h->stream));
+            if ((rc = pos_step(h, steps))) return rc;
+            HIP_TRY(hipEventRecord(e1, h->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            const double per_step = ms / steps;
+            if (per_step < best_t) { best_t = per_step; best = o; }
+            if ((rc = restore())) return rc;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    best.use_graph = keep_graph;
+    if ((rc = build_launch(h, best))) return rc;
+    if (chosen) *chosen = h->opts;
+    return POS_OK;
+}
+
+void pos_destroy(pos_ens *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    for (void *p : h->allocs) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // extern "C"

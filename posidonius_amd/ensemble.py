@@ -1,0 +1,113 @@
+"""Python handle over the C ABI: an ensemble of WHFast systems on one GPU.
+
+The object mirrors the reference's ``Integrator`` trait
+(src/integrator/mod.rs:16-26) for a batch of universes: ``step`` is
+``iterate`` applied ``n`` times to every system, ``set_time_limit`` and
+``set_historic_snapshot_period`` are the trait's setters, and the state
+accessors expose what ``Universe.particles`` holds.  All work happens in
+``libposidonius_hip.so``; if the library is absent, constructing an
+``Ensemble`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import abi
+from .case import EnsembleSpec
+
+MAPPINGS = {"auto": abi.MAPPING_AUTO, "plain": abi.MAPPING_PLAIN, "packed": abi.MAPPING_PACKED}
+
+
+class Ensemble:
+    def __init__(self, spec: EnsembleSpec, mapping: str = "packed", group_size: int = 0,
+                 steps_per_launch: int = 0, use_graph: bool = False, device: int = 0,
+                 persistent_grid: int = 0):
+        self._lib = abi.library()
+        self.spec = spec
+        self._holder = spec.desc()
+        opts = abi.Options()
+        opts.device = device
+        opts.mapping = MAPPINGS[mapping] if isinstance(mapping, str) else int(mapping)
+        opts.group_size = group_size
+        opts.steps_per_launch = steps_per_launch
+        opts.use_graph = 1 if use_graph else 0
+        opts.reserved[0] = persistent_grid
+        h = C.c_void_p()
+        abi.check(self._lib.pos_create(self._holder.ref, C.byref(opts), C.byref(h)))
+        self._h = h
+        self.n_systems = spec.n_systems
+        self.n_bodies = spec.n_bodies
+
+    # -- lifecycle ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pos_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- stepping -----------------------------------------------------------
+    def step(self, n_steps: int = 1):
+        abi.check(self._lib.pos_step(self._h, C.c_int64(int(n_steps))))
+
+    def synchronize(self):
+        abi.check(self._lib.pos_synchronize(self._h))
+
+    def tune(self, probe_steps: int = 32) -> abi.Options:
+        o = abi.Options()
+        abi.check(self._lib.pos_tune(self._h, int(probe_steps), C.byref(o)))
+        return o
+
+    def stream(self) -> int:
+        """hipStream_t of the handle (as an integer address)."""
+        st = C.c_void_p()
+        abi.check(self._lib.pos_get_stream(self._h, C.byref(st)))
+        return st.value or 0
+
+    def options(self) -> abi.Options:
+        o = abi.Options()
+        abi.check(self._lib.pos_get_options(self._h, C.byref(o)))
+        return o
+
+    def set_time_limit(self, time_limit: float):
+        abi.check(self._lib.pos_set_time_limit(self._h, float(time_limit)))
+
+    def set_historic_snapshot_period(self, period: float):
+        abi.check(self._lib.pos_set_historic_snapshot_period(self._h, float(period)))
+
+    # -- state ----------------------------------------------------------------
+    def get_state(self, fields=None) -> Dict[str, np.ndarray]:
+        st, arrays = abi.alloc_state(self.n_systems, self.n_bodies, fields)
+        abi.check(self._lib.pos_get_state(self._h, C.byref(st)))
+        return arrays
+
+    def set_state(self, arrays: Dict[str, np.ndarray]):
+        arrays = dict(arrays)
+        st = abi.state_struct(arrays)
+        abi.check(self._lib.pos_set_state(self._h, C.byref(st)))
+
+    # -- operator-level evaluations ------------------------------------------
+    def gravity_acceleration(self):
+        acc = np.zeros((self.n_bodies, 3))
+        status = np.zeros(self.n_systems, dtype=np.uint32)
+        abi.check(self._lib.pos_gravity_acceleration(self._h, abi.ptr(acc, C.c_double), abi.ptr(status, C.c_uint32)))
+        return acc, status
+
+    def additional_effects(self):
+        acc = np.zeros((self.n_bodies, 3))
+        dl = np.zeros((self.n_bodies, 3))
+        abi.check(self._lib.pos_additional_effects(self._h, abi.ptr(acc, C.c_double), abi.ptr(dl, C.c_double)))
+        return acc, dl

@@ -1,0 +1,183 @@
+"""Parity of the HIP path with the CPU oracle and the reference's fixtures.
+
+Every comparison here is bitwise: the device code and the oracle perform the
+same IEEE binary64 operations in the same order (both built without
+multiply-add contraction), so a single differing bit is a failure.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from posidonius_amd import abi, generators as gen
+from posidonius_amd.case import EnsembleSpec
+from tests.oracle_runner import golden_names, load_golden
+
+pytestmark = pytest.mark.gpu
+
+UNSUPPORTED = {"test_disk-enabled_disk", "test_general_relativity-anderson1975",
+               "test_general_relativity-newhall1983", "test_evolution-solar_like_bolmontmathis2016",
+               "test_evolution-solar_like_galletbolmont2017"}
+SUPPORTED = [n for n in golden_names() if n not in UNSUPPORTED]
+STATE_KEYS = ("position", "velocity", "acceleration", "angular_momentum", "spin", "velocity_error",
+              "angular_momentum_error", "radius", "moment_of_inertia", "current_time", "current_iteration",
+              "status")
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def Ensemble():
+    from posidonius_amd.ensemble import Ensemble
+    return Ensemble
+
+
+def assert_same_state(gpu, ref, keys=STATE_KEYS, where=""):
+    for k in keys:
+        a, b = np.asarray(gpu[k]), np.asarray(ref[k])
+        if not np.array_equal(a, b):
+            bad = np.argwhere(a != b)
+            i = tuple(bad[0])
+            diff = np.max(np.abs(a.astype(np.float64) - b.astype(np.float64)))
+            raise AssertionError(f"{where} {k}: {len(bad)} entries differ, first at {i}: gpu={a[i]!r} "
+                                 f"oracle={b[i]!r}, max abs diff {diff:.3e}")
+
+
+def test_fixtures_one_ensemble_bitwise(Ensemble):
+    """All 21 supported reference fixtures in one packed ensemble, run to
+    completion (199 steps): positions, velocities and accelerations equal the
+    reference's particle_<i>.json bit for bit."""
+    specs, expected = [], []
+    for name in SUPPORTED:
+        s, e = load_golden(name)
+        specs.append(s)
+        expected.append(e)
+    spec = EnsembleSpec.concat(specs)
+    log(f"fixtures ensemble: {spec.n_systems} systems, {spec.n_bodies} bodies")
+    with Ensemble(spec, mapping="packed") as ens:
+        ens.step(250)
+        out = ens.get_state()
+    assert np.all(out["current_iteration"] == 199)
+    assert np.all(out["status"] & abi.STATUS_COMPLETED)
+    for s, name in enumerate(SUPPORTED):
+        b0, b1 = spec.body_offset[s], spec.body_offset[s + 1]
+        for k in ("position", "velocity", "acceleration"):
+            np.testing.assert_array_equal(out[k][b0:b1], expected[s][k], err_msg=f"{name} {k}")
+
+
+@pytest.mark.parametrize("mapping", ["plain", "packed"])
+def test_fixtures_against_oracle_full_state(Ensemble, oracle, mapping):
+    names = SUPPORTED[::4]
+    spec = EnsembleSpec.concat([load_golden(n)[0] for n in names])
+    ref = oracle.run(spec, 120)
+    log(f"fixtures vs oracle mapping={mapping}: {spec.n_systems} systems x 120 steps")
+    with Ensemble(spec, mapping=mapping) as ens:
+        ens.step(120)
+        out = ens.get_state()
+    assert_same_state(out, ref, where=f"[{mapping}]")
+
+
+@pytest.mark.parametrize("cfg,n,steps", [(0, None, 1000), (1, 256, 1000), (2, 512, 300), (3, 128, 300), (4, 64, 1000)])
+def test_configs_bitwise_against_oracle(Ensemble, oracle, cfg, n, steps):
+    spec = gen.CONFIGS[cfg]() if n is None else gen.CONFIGS[cfg](n)
+    log(f"config {cfg}: {spec.n_systems} systems x {steps} steps")
+    ref = oracle.run(spec, steps, threads=8)
+    with Ensemble(spec, mapping="packed", steps_per_launch=64) as ens:
+        ens.step(steps)
+        out = ens.get_state()
+    assert_same_state(out, ref, where=f"config {cfg}")
+
+
+@pytest.mark.parametrize("group_size,K,graph", [(0, 1, False), (0, 7, False), (8, 64, True), (16, 32, False), (3, 16, True)])
+def test_launch_mappings_agree(Ensemble, oracle, group_size, K, graph):
+    """Group size, steps per launch and graph replay do not change a bit."""
+    spec = gen.config1(96) if group_size != 16 else gen.config2(96)
+    if group_size == 3:
+        spec = gen.config1(96)
+    steps = 100
+    ref = oracle.run(spec, steps, threads=8)
+    log(f"mapping G={group_size} K={K} graph={graph}")
+    with Ensemble(spec, mapping="packed", group_size=group_size, steps_per_launch=K, use_graph=graph) as ens:
+        ens.step(steps)
+        out = ens.get_state()
+    assert_same_state(out, ref, where=f"G={group_size} K={K} graph={graph}")
+
+
+def test_operators_against_oracle(Ensemble, oracle):
+    """pos_gravity_acceleration / pos_additional_effects: the two operator-level
+    entry points the reference benchmarks (benches/simulation.rs:127, 138)."""
+    spec = EnsembleSpec.concat([gen.config2(64), gen.config3(16), load_golden("test_integrator-whfast_jacobi")[0]])
+    with Ensemble(spec) as ens:
+        g_acc, g_st = ens.gravity_acceleration()
+        e_acc, e_dl = ens.additional_effects()
+    r_acc, r_st = oracle.gravity(spec)
+    np.testing.assert_array_equal(g_acc, r_acc)
+    np.testing.assert_array_equal(g_st, r_st)
+    r_eacc, r_edl = oracle.additional_effects(spec)
+    np.testing.assert_array_equal(e_acc, r_eacc)
+    np.testing.assert_array_equal(e_dl, r_edl)
+
+
+def test_resume_equals_continuous_run(Ensemble, oracle):
+    """get_state -> new ensemble -> continue gives the same bits as one run
+    (checkpoint/resume, src/integrator/output.rs:28-54, 178-203)."""
+    spec = gen.config1(64)
+    with Ensemble(spec) as ens:
+        ens.step(300)
+        mid = ens.get_state()
+        ens.step(200)
+        full = ens.get_state()
+    resumed = spec.copy()
+    for k in ("position", "velocity", "angular_momentum", "spin", "velocity_error", "angular_momentum_error"):
+        resumed.body[k] = mid[k].copy()
+    resumed.system["current_time"] = mid["current_time"].copy()
+    resumed.system["last_historic_snapshot_time"] = mid["last_historic_snapshot_time"].copy()
+    resumed.system["current_iteration"] = mid["current_iteration"].copy()
+    with Ensemble(resumed) as ens2:
+        ens2.set_state({"acceleration": mid["acceleration"]})
+        ens2.step(200)
+        out = ens2.get_state()
+    assert_same_state(out, full, keys=("position", "velocity", "angular_momentum", "spin", "velocity_error",
+                                       "angular_momentum_error", "current_time", "current_iteration"))
+
+
+def test_collision_freezes_system(Ensemble, oracle):
+    """A collision (universe.rs:230-233) raises POS_STATUS_COLLISION and
+    freezes that system only; the oracle agrees bit for bit."""
+    spec = gen.config1(8)
+    b = spec.body_offset[3] + 1
+    spec.body["radius"][b] = 0.2          # inflate one planet past the host
+    ref = oracle.run(spec, 50)
+    with Ensemble(spec) as ens:
+        ens.step(50)
+        out = ens.get_state()
+    assert out["status"][3] & abi.STATUS_COLLISION
+    assert out["current_iteration"][3] == 1
+    assert np.all(out["current_iteration"][np.arange(8) != 3] == 50)
+    assert_same_state(out, ref, where="collision")
+
+
+def test_large_ensemble_subset_properties(Ensemble, oracle):
+    """At the benchmark size the oracle is too slow for every system; check
+    that (a) a random subset equals the oracle bit for bit and (b) the run is
+    deterministic across launches."""
+    spec = gen.config2(16384)
+    steps = 64
+    with Ensemble(spec, steps_per_launch=32) as ens:
+        ens.step(steps)
+        out = ens.get_state()
+    with Ensemble(spec, steps_per_launch=16, group_size=16) as ens:
+        ens.step(steps)
+        out2 = ens.get_state()
+    assert_same_state(out2, out, where="determinism")
+    rng = np.random.default_rng(7)
+    pick = np.sort(rng.choice(spec.n_systems, 64, replace=False))
+    sub = spec.subset(pick)
+    ref = oracle.run(sub, steps, threads=8)
+    idx = np.concatenate([np.arange(spec.body_offset[s], spec.body_offset[s + 1]) for s in pick])
+    sub_out = {k: (v[pick] if v.shape[0] == spec.n_systems else v[idx]) for k, v in out.items()}
+    assert_same_state(sub_out, ref, where="subset")
