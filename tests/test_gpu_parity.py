@@ -150,6 +150,7 @@ def test_collision_freezes_system(Ensemble, oracle):
     freezes that system only; the oracle agrees bit for bit."""
     spec = gen.config1(8)
     b = spec.body_offset[3] + 1
+    spec.system["effects"][3] = 0          # point masses: the radius enters only the collision test
     spec.body["radius"][b] = 0.2          # inflate one planet past the host
     ref = oracle.run(spec, 50)
     with Ensemble(spec) as ens:
