@@ -214,7 +214,7 @@ def state_struct(arrays: dict):
 # ---------------------------------------------------------------------------
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_NAME = "libposidonius_hip.so"
-LIB_PATH = os.path.join(_HERE, LIB_NAME)
+LIB_PATH = os.environ.get("POSIDONIUS_HIP_LIBRARY") or os.path.join(_HERE, LIB_NAME)
 _lib = None
 
 

@@ -18,6 +18,10 @@
 
 #include "whfast_step.hpp"
 
+#ifndef POS_WAVES_PER_EU
+#define POS_WAVES_PER_EU 2   // occupancy target of the step kernel (waves per SIMD)
+#endif
+
 using namespace posdev;
 
 // ===========================================================================
@@ -38,100 +42,88 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
     B.k = k;
     B.sys = s;
     B.n = gvalid ? (E.body_offset[s + 1] - E.body_offset[s]) : 0;
-    B.valid = gvalid && k < B.n;
+    const bool valid = gvalid && k < B.n;
     const int b0 = E.body_offset[s];
     B.host = gvalid ? E.host[s] : 0;   // keeps gbase + host inside this wave's LDS rows
-    B.is_host = B.valid && (k == B.host);
-    B.body = b0 + (B.valid ? k : 0);
+    B.body = b0 + (valid ? k : 0);
     B.coord = E.coord[s];
-    B.eff = E.effects[s];
-    B.tides = B.eff & POS_EFFECT_TIDES;
-    B.rf = B.eff & POS_EFFECT_ROTATIONAL_FLATTENING;
-    B.gr = B.eff & POS_EFFECT_GENERAL_RELATIVITY;
-    B.wind = B.eff & POS_EFFECT_WIND;
-    B.evolution = B.eff & POS_EFFECT_EVOLUTION;
-    B.spin_on = B.tides || B.rf || B.evolution || B.gr;
+    const uint32_t eff = E.effects[s];
+    const int b = B.body;
+    const uint32_t roles = E.roles[b];
+    B.evo = E.evo_type[b];
+    uint32_t f = 0;
+    if (valid) f |= F_VALID;
+    if (valid && k == B.host) f |= F_HOST;
+    if (eff & POS_EFFECT_TIDES) f |= F_TIDES;
+    if (eff & POS_EFFECT_ROTATIONAL_FLATTENING) f |= F_RF;
+    if (eff & POS_EFFECT_GENERAL_RELATIVITY) f |= F_GR;
+    if (eff & POS_EFFECT_WIND) f |= F_WIND;
+    if (eff & POS_EFFECT_EVOLUTION) f |= F_EVOLUTION;
+    if (eff & (POS_EFFECT_TIDES | POS_EFFECT_ROTATIONAL_FLATTENING | POS_EFFECT_EVOLUTION | POS_EFFECT_GENERAL_RELATIVITY))
+        f |= F_SPIN;
+    if (roles & POS_ROLE_WIND) f |= F_WIND_ROLE;
+    if (B.evo == POS_EVO_LECONTE_CHABRIER2013 && (roles & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_TIDES_ORBITING)))
+        f |= F_LC_LOVE;
+    B.flags = f;
     B.dt = E.dt[s];
-    B.half_dt = 0.5 * B.dt;
-    B.T = E.time_limit[s];
     B.t = E.t[s];
     B.last_hist = E.last_hist[s];
-    B.hist_period = E.hist_period[s];
     B.iteration = E.iteration[s];
     B.status = E.status[s];
-    if (!B.valid) B.status |= POS_STATUS_COMPLETED;   // idle lanes never iterate
+    if (!valid) B.status |= POS_STATUS_COMPLETED;   // idle lanes never iterate
     B.tw = E.timestep_warning[s];
-    const int b = B.body;
+    B.left_index = E.left_index[b];
     const int nb = E.n_bodies;
     B.m = E.mass[b];
-    B.mg = B.m * K2;
-    B.R = E.radius[b];
-    B.rg2 = E.rg2[b];
-    B.I = E.inertia[b];
-    B.sigma = E.sigma[b];
-    B.k2 = E.k2[b];
-    B.k2f = E.k2f[b];
-    B.wk = E.wind_k[b];
-    B.wsat = E.wind_sat[b];
-    B.wsat2 = B.wsat * B.wsat;
-    B.roles = E.roles[b];
+    B.inv_m = 1. / B.m;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         B.x[q] = E.x[q * nb + b];
         B.v[q] = E.v[q * nb + b];
         B.L[q] = E.L[q * nb + b];
-        B.s[q] = E.spin[q * nb + b];
         B.ev[q] = E.err_v[q * nb + b];
         B.eL[q] = E.err_L[q * nb + b];
-        B.ag[q] = E.acc[q * nb + b];
     }
-    B.s2 = p2(B.s[0]) + p2(B.s[1]) + p2(B.s[2]);
-    const int hb = b0 + B.host;
-    B.hm = E.mass[hb];
-    B.hmg = B.hm * K2;
-    B.hsigma = E.sigma[hb];
-    B.hk2 = E.k2[hb];
-    B.hk2f = E.k2f[hb];
-    B.evo = E.evo_type[b];
-    B.evo_off = E.evo_off[b];
-    B.evo_len = E.evo_len[b];
-    B.left_index = E.left_index[b];
-    lds.m[lane] = B.valid ? B.m : 0.;
-    lds.mg[lane] = B.valid ? B.mg : 0.;
-    __syncthreads();
-    // Jacobi prefix constants (whfast.rs:574-576, 642-651)
-    double mu = B.hmg, eta = B.hm;
-    int rank = 0, my_rank = 0;
-    for (int j = 0; j < B.n; ++j) {
-        if (j == B.host) continue;
-        mu = mu + lds.mg[B.gbase + j];
-        eta = eta + lds.m[B.gbase + j];
-        if (j == k) { B.jac_mu = mu; B.jac_eta = eta; my_rank = rank; }
-        ++rank;
-    }
-    B.jac_rank = my_rank;
+    // persistent LDS area
+    lds.m[lane] = valid ? B.m : 0.;
+    lds.mg[lane] = valid ? B.m * K2 : 0.;
+    lds.inv_m[lane] = B.inv_m;
+    lds.sigma[lane] = E.sigma[b];
+    lds.k2[lane] = E.k2[b];
+    lds.k2f[lane] = E.k2f[b];
+    lds.R[lane] = E.radius[b];
+    lds.I[lane] = E.inertia[b];
+    double sp[3];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) { B.ax[q] = 0.; B.av[q] = 0.; B.aacc[q] = 0.; B.comx[q] = 0.; B.comv[q] = 0.; }
+    for (int q = 0; q < 3; ++q) {
+        sp[q] = E.spin[q * nb + b];
+        lds.s[q][lane] = sp[q];
+        lds.ag[q][lane] = E.acc[q * nb + b];
+    }
+    lds.s2[lane] = p2(sp[0]) + p2(sp[1]) + p2(sp[2]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) { B.ax[q] = 0.; B.av[q] = 0.; B.comx[q] = 0.; B.comv[q] = 0.; }
     B.com_m = 0.;
+    __syncthreads();
 }
 
-__device__ inline void store_lane(const DevEnsemble &E, const Lane &B) {
-    if (!B.valid) return;
+__device__ inline void store_lane(const DevEnsemble &E, const WaveLds &lds, const Lane &B) {
+    if (!B.valid()) return;
     const int b = B.body;
     const int nb = E.n_bodies;
+    const int lane = B.lane;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         E.x[q * nb + b] = B.x[q];
         E.v[q * nb + b] = B.v[q];
         E.L[q * nb + b] = B.L[q];
-        E.spin[q * nb + b] = B.s[q];
+        E.spin[q * nb + b] = lds.s[q][lane];
         E.err_v[q * nb + b] = B.ev[q];
         E.err_L[q * nb + b] = B.eL[q];
-        E.acc[q * nb + b] = B.ag[q];
+        E.acc[q * nb + b] = lds.ag[q][lane];
     }
-    E.radius[b] = B.R;
-    E.rg2[b] = B.rg2;
-    E.inertia[b] = B.I;
+    E.radius[b] = lds.R[lane];
+    E.inertia[b] = lds.I[lane];
     E.left_index[b] = B.left_index;
     if (B.k == 0) {
         const int s = B.sys;
@@ -148,7 +140,8 @@ constexpr uint32_t FROZEN = POS_STATUS_COMPLETED | POS_STATUS_FATAL;
 // K steps of every system of every wave; the grid may be smaller than the
 // number of waves, in which case each workgroup walks the table with a
 // grid stride (persistent launch).
-__global__ __launch_bounds__(64) void whfast_steps_kernel(DevEnsemble E, int n_steps) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POS_WAVES_PER_EU)))
+void whfast_steps_kernel(DevEnsemble E, int n_steps) {
     __shared__ WaveLds lds;
     for (int wave = blockIdx.x; wave < E.n_waves; wave += gridDim.x) {
         Lane B;
@@ -157,7 +150,7 @@ __global__ __launch_bounds__(64) void whfast_steps_kernel(DevEnsemble E, int n_s
             if (B.status & FROZEN) break;   // uniform within the group
             iterate(E, lds, B);
         }
-        store_lane(E, B);
+        store_lane(E, lds, B);
         __syncthreads();
     }
 }
@@ -171,10 +164,10 @@ __global__ __launch_bounds__(64) void init_kernel(DevEnsemble E) {
     Lane B;
     load_lane(E, lds, B, wave);
     if (B.t == 0.) {
-        if (B.evolution) evolve_body(E, B, B.t);
-        own_spin(B);
+        if (B.flags & F_EVOLUTION) evolve_body(E, lds, B, B.t);
+        own_spin(lds, B);
         merge_status(lds, B);
-        store_lane(E, B);
+        store_lane(E, lds, B);
     }
 }
 
@@ -188,9 +181,9 @@ __global__ __launch_bounds__(64) void gravity_kernel(DevEnsemble E, double *out_
     gravity(E, lds, B);
     merge_status(lds, B);
     const int nb = E.n_bodies;
-    if (B.valid) {
+    if (B.valid()) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) out_acc[q * nb + B.body] = B.ag[q];
+        for (int q = 0; q < 3; ++q) out_acc[q * nb + B.body] = lds.ag[q][B.lane];
         if (B.k == 0) out_status[B.sys] = B.status;
     }
 }
@@ -204,7 +197,7 @@ __global__ __launch_bounds__(64) void effects_kernel(DevEnsemble E, double *out_
     double a[3], dL[3];
     additional_effects(E, lds, B, false, a, dL);
     const int nb = E.n_bodies;
-    if (B.valid) {
+    if (B.valid()) {
 #pragma unroll
         for (int q = 0; q < 3; ++q) { out_acc[q * nb + B.body] = a[q]; out_dLdt[q * nb + B.body] = dL[q]; }
     }

@@ -1,14 +1,23 @@
 // whfast_step.hpp -- device implementation of one WHFast step for a lane group.
 //
 // Mapping: a wavefront of 64 lanes holds 64/G systems; the G lanes of a group
-// hold the bodies of one system, lane k <-> body k (k < n).  Every body-local
-// quantity lives in the lane's registers.  Quantities that couple bodies go
-// through a per-wavefront LDS exchange area:
-//   * sums over bodies (centre of mass, momentum, host reactions, the
-//     implicit-midpoint convergence norms) are re-accumulated by every lane
-//     of the group from the exchange area, strictly in body-index order, so
-//     the result is the same rounded value the sequential reference computes;
-//   * the host's state is read by the orbiting bodies from the same area.
+// hold the bodies of one system, lane k <-> body k (k < n).
+//
+// Storage per lane:
+//   * registers: the hot state (position, velocity, spin angular momentum,
+//     compensated-summation errors, mass, time step);
+//   * a persistent per-wavefront LDS area: per-body parameters, the spin of
+//     the last evaluation (the reference reads it "stale", see
+//     additional_effects), radius, moment of inertia and the gravitational
+//     acceleration.  Orbiting bodies read their host's values from the host
+//     lane's slot;
+//   * global memory: per-system bookkeeping touched once per step.
+// Quantities that couple bodies go through an exchange area of the same LDS
+// block.  Sums over bodies (centre of mass, momentum, host reactions, the
+// implicit-midpoint convergence norms) are re-accumulated strictly in
+// body-index order, so every result is the same rounded value the sequential
+// reference computes.
+//
 // All arithmetic is binary64 and this file must be compiled with
 // -ffp-contract=off so that each operation rounds exactly as in the
 // reference (src/integrator/whfast.rs, src/particles/universe.rs,
@@ -52,6 +61,26 @@ __device__ __forceinline__ double p7(double x) { double x2 = x * x; double x4 = 
 __device__ __forceinline__ double p8(double x) { double x2 = x * x; double x4 = x2 * x2; return x4 * x4; }
 __device__ __forceinline__ double p10(double x) { double x2 = x * x; double x4 = x2 * x2; double x8 = x4 * x4; return x2 * x8; }
 
+// ---- division by a shared divisor ------------------------------------------
+// Many quotients of one evaluation share a divisor (the distance, |v|, a mass,
+// the moment of inertia).  For a divisor b with correctly rounded reciprocal
+// r = RN(1/b), q0 = RN(a*r) is within one ulp of a/b, the residual
+// e = a - b*q0 is exact in one FMA, and RN(q0 + e*r) is the correctly rounded
+// quotient RN(a/b) (Markstein's theorem; Muller et al., Handbook of
+// Floating-Point Arithmetic, 2nd ed., sec. 4.7), barring overflow and
+// underflow, which the step's magnitudes are far from.  The result is thus
+// bit-identical to a/b at about a third of the instruction cost.  An exact
+// zero residual keeps q0, which preserves the sign of a zero quotient.
+struct Rcp {
+    double b, r;
+};
+__device__ __forceinline__ Rcp rcp(double b) { return Rcp{b, 1.0 / b}; }
+__device__ __forceinline__ double divr(double a, const Rcp &R) {
+    const double q0 = a * R.r;
+    const double e = __builtin_fma(-R.b, q0, a);
+    return e == 0. ? q0 : __builtin_fma(e, R.r, q0);
+}
+
 // ---- device view of an ensemble (structure of arrays, device order) -------
 struct DevEnsemble {
     int n_systems, n_bodies;
@@ -80,41 +109,45 @@ struct DevEnsemble {
     int n_waves;
 };
 
-// ---- LDS exchange area of one wavefront -------------------------------------
-constexpr int NX = 26;   // exchange slots per lane
+// ---- LDS of one wavefront ---------------------------------------------------
+constexpr int NX = 24;   // exchange slots per lane
 struct WaveLds {
-    double m[WAVE], mg[WAVE];     // constant per body
-    double xb[NX][WAVE];          // phase-dependent exchange
+    // persistent per body (lane)
+    double m[WAVE], mg[WAVE], inv_m[WAVE], sigma[WAVE], k2[WAVE], k2f[WAVE];
+    double R[WAVE], I[WAVE];
+    double s[3][WAVE], s2[WAVE];   // spin of the last evaluation and its squared norm
+    double ag[3][WAVE];            // gravitational acceleration (universe.rs:300-302)
+    // phase-dependent exchange
+    double xb[NX][WAVE];
 };
 
 __device__ __forceinline__ void wsync() { __syncthreads(); }
 
-// ---- per-lane state ---------------------------------------------------------
+// ---- per-lane registers -----------------------------------------------------
+enum : uint32_t {
+    F_VALID = 1u << 0, F_HOST = 1u << 1, F_TIDES = 1u << 2, F_RF = 1u << 3, F_GR = 1u << 4,
+    F_WIND = 1u << 5, F_EVOLUTION = 1u << 6, F_SPIN = 1u << 7, F_WIND_ROLE = 1u << 8, F_LC_LOVE = 1u << 9
+};
 struct Lane {
-    // identity
     int lane, gbase, k, n, host, coord, sys, body;
-    bool valid, is_host;
-    uint32_t eff, roles;
-    bool tides, rf, gr, wind, evolution, spin_on;
-    // system scalars (identical copies in every lane of the group)
-    double t, dt, half_dt, T, last_hist, hist_period;
+    uint32_t flags, status;
+    int tw, evo, left_index;
     long long iteration;
-    uint32_t status;
-    int tw;
-    // body
-    double m, mg, R, rg2, I, sigma, k2, k2f, wk, wsat, wsat2;
-    double x[3], v[3], L[3], s[3], s2, ev[3], eL[3], ag[3];
-    // host constants (masses never change)
-    double hm, hmg, hsigma, hk2, hk2f;
-    // Jacobi constants
-    double jac_mu;      // G * cumulative mass up to this body (whfast.rs:642-651)
-    double jac_eta;     // cumulative mass up to this body (whfast.rs:574-576)
-    int jac_rank;       // position among the non-host bodies
-    // evolution
-    int evo, evo_off, evo_len, left_index;
+    double t, dt, last_hist;
+    double m, inv_m;
+    double x[3], v[3], L[3], ev[3], eL[3];
     // alternative coordinates of this body; com* hold the host slot (centre of mass)
-    double ax[3], av[3], aacc[3];
+    double ax[3], av[3];
     double comx[3], comv[3], com_m;
+
+    __device__ __forceinline__ bool valid() const { return flags & F_VALID; }
+    __device__ __forceinline__ bool is_host() const { return flags & F_HOST; }
+    __device__ __forceinline__ bool orbiting() const { return (flags & (F_VALID | F_HOST)) == F_VALID; }
+    __device__ __forceinline__ bool tides() const { return flags & F_TIDES; }
+    __device__ __forceinline__ bool rf() const { return flags & F_RF; }
+    __device__ __forceinline__ bool gr() const { return flags & F_GR; }
+    __device__ __forceinline__ bool spin_on() const { return flags & F_SPIN; }
+    __device__ __forceinline__ int hl() const { return gbase + host; }
 };
 
 // ===========================================================================
@@ -147,54 +180,49 @@ __device__ __forceinline__ int evolver_idx(int left_index, int len) {
     return i0 < len ? i0 : len - 1;
 }
 
-__device__ inline void evolve_body(const DevEnsemble &E, Lane &B, double t) {
-    if (!B.valid || B.evo == POS_EVO_NON_EVOLVING) return;
-    const double *tt = E.tab_time + B.evo_off;
-    int i0 = evolver_idx(B.left_index, B.evo_len);
+// calculate_particles_non_spin_dependent_evolving_quantities for this body
+__device__ inline void evolve_body(const DevEnsemble &E, WaveLds &lds, Lane &B, double t) {
+    if (!B.valid() || B.evo == POS_EVO_NON_EVOLVING) return;
+    const int off = E.evo_off[B.body], len = E.evo_len[B.body];
+    const double *tt = E.tab_time + off;
+    int i0 = evolver_idx(B.left_index, len);
     int left = 0;
-    double newR = interp_table(tt + i0, E.tab_radius + B.evo_off + i0, B.evo_len - i0, t, &left);
+    const double R = lds.R[B.lane], rg2 = E.rg2[B.body];
+    const double newR = interp_table(tt + i0, E.tab_radius + off + i0, len - i0, t, &left);
     B.left_index += left;
-    double newrg2 = B.rg2;
+    double newrg2 = rg2;
     if (B.evo == POS_EVO_BARAFFE2015 || B.evo == POS_EVO_LECONTE2011 || B.evo == POS_EVO_LECONTE_CHABRIER2013) {
-        i0 = evolver_idx(B.left_index, B.evo_len);
-        newrg2 = interp_table(tt + i0, E.tab_rg2 + B.evo_off + i0, B.evo_len - i0, t, &left);
+        i0 = evolver_idx(B.left_index, len);
+        newrg2 = interp_table(tt + i0, E.tab_rg2 + off + i0, len - i0, t, &left);
         B.left_index += left;
     }
-    if (newR != B.R || newrg2 != B.rg2) {
-        B.R = newR;
-        B.rg2 = newrg2;
-        B.I = B.m * B.rg2 * p2(B.R);
+    if (newR != R || newrg2 != rg2) {
+        lds.R[B.lane] = newR;
+        E.rg2[B.body] = newrg2;
+        lds.I[B.lane] = B.m * newrg2 * p2(newR);
     }
-    if (B.evo == POS_EVO_LECONTE_CHABRIER2013 && (B.roles & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_TIDES_ORBITING))) {
-        i0 = evolver_idx(B.left_index, B.evo_len);
-        (void)interp_table(tt + i0, E.tab_love + B.evo_off + i0, B.evo_len - i0, t, &left);
+    if (B.flags & F_LC_LOVE) {   // love number evaluated and discarded (evolution.rs:536-538)
+        i0 = evolver_idx(B.left_index, len);
+        (void)interp_table(tt + i0, E.tab_love + off + i0, len - i0, t, &left);
         B.left_index += left;
     }
 }
 
 // particles/common.rs:3-14 for this lane's body
-__device__ __forceinline__ void own_spin(Lane &B) {
-    if (!B.valid) return;
-    if (B.I == 0.) { B.status |= POS_STATUS_ZERO_INERTIA; return; }
-    B.s[0] = B.L[0] / B.I;
-    B.s[1] = B.L[1] / B.I;
-    B.s[2] = B.L[2] / B.I;
-    B.s2 = p2(B.s[0]) + p2(B.s[1]) + p2(B.s[2]);
+__device__ __forceinline__ void own_spin(WaveLds &lds, Lane &B) {
+    if (!B.valid()) return;
+    const double I = lds.I[B.lane];
+    if (I == 0.) { B.status |= POS_STATUS_ZERO_INERTIA; return; }
+    const Rcp RI = rcp(I);
+    const double s0 = divr(B.L[0], RI), s1 = divr(B.L[1], RI), s2 = divr(B.L[2], RI);
+    lds.s[0][B.lane] = s0; lds.s[1][B.lane] = s1; lds.s[2][B.lane] = s2;
+    lds.s2[B.lane] = p2(s0) + p2(s1) + p2(s2);
 }
-
-// The zero-inertia flag is raised by whichever lane meets it; every lane of
-// the group must agree on the status word, so it is OR-reduced through LDS
-// at the end of each step.
 
 // ===========================================================================
 // Additional effects + implicit midpoint (whfast.rs:322-466, universe.rs:353-614)
 // ===========================================================================
-// Exchange slots used inside one implicit-midpoint iteration.
-enum {
-    X_X0 = 0, X_X1, X_X2, X_V0, X_V1, X_V2, X_S0, X_S1, X_S2,   // round 1
-    X_FS0 = 9, X_FS1, X_FS2, X_FS2N, X_R, X_I, X_L0, X_L1, X_L2  // round 2
-};
-enum {   // round 3: orbiting body -> host contributions
+enum {   // orbiting body -> host contributions
     C_TF = 0,    // tidal force (3)
     C_RF = 3,    // flattening force (3)
     C_TN = 6,    // -1 * tidal torque on the host (3)
@@ -206,337 +234,370 @@ enum {   // round 3: orbiting body -> host contributions
     C_COUNT = 24
 };
 
+__device__ __forceinline__ void put3(WaveLds &lds, int slot, int lane, double x, double y, double z) {
+    lds.xb[slot][lane] = x; lds.xb[slot + 1][lane] = y; lds.xb[slot + 2][lane] = z;
+}
+
+// Ordered sum over the orbiting bodies of the group of a 3-slot contribution.
+__device__ __forceinline__ void host_sum3(const WaveLds &lds, const Lane &B, int slot, double out[3]) {
+    double s0 = 0., s1 = 0., s2 = 0.;
+    for (int j = 0; j < B.n; ++j) {
+        if (j == B.host) continue;
+        const int l = B.gbase + j;
+        s0 = s0 + lds.xb[slot][l];
+        s1 = s1 + lds.xb[slot + 1][l];
+        s2 = s2 + lds.xb[slot + 2][l];
+    }
+    out[0] = s0; out[1] = s1; out[2] = s2;
+}
+
+// Accelerations and spin derivatives of every body of the group at the current
+// (averaged) state.  Outputs are accumulated in the reference's order:
+// a_add = ((0 + tides) + flattening) + GR and
+// dL/dt = ((tides + flattening) + GR) + wind (universe.rs:540-614).
 __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, Lane &B, bool evolution,
                                           double a_add[3], double dLdt[3]) {
     const int lane = B.lane;
-    // ---- round 1: inertial state and the (stale) spin of every body ----
-    if (B.valid) {
-        lds.xb[X_X0][lane] = B.x[0]; lds.xb[X_X1][lane] = B.x[1]; lds.xb[X_X2][lane] = B.x[2];
-        lds.xb[X_V0][lane] = B.v[0]; lds.xb[X_V1][lane] = B.v[1]; lds.xb[X_V2][lane] = B.v[2];
-        lds.xb[X_S0][lane] = B.s[0]; lds.xb[X_S1][lane] = B.s[1]; lds.xb[X_S2][lane] = B.s[2];
+    const int hl = B.hl();
+    const bool orbiting = B.orbiting();
+    // ---- inertial -> heliocentric (universe.rs:318-351); the host's
+    //      heliocentric velocity is zero at this point ----
+    if (B.valid()) {
+        lds.xb[0][lane] = B.x[0]; lds.xb[1][lane] = B.x[1]; lds.xb[2][lane] = B.x[2];
+        lds.xb[3][lane] = B.v[0]; lds.xb[4][lane] = B.v[1]; lds.xb[5][lane] = B.v[2];
     }
     wsync();
-    const int hl = B.gbase + B.host;
     double px[3] = {0., 0., 0.}, pv[3] = {0., 0., 0.};
-    double d = 0., vr = 0., nv2 = 0., nv = 0.;
+    double d = 1., nv2 = 0., nv = 1.;
     double rs_star = 0., rs_planet = 0.;
-    if (B.valid && !B.is_host) {
-        // universe.rs:318-351 (the host's heliocentric velocity is zero here)
-        px[0] = B.x[0] - lds.xb[X_X0][hl]; px[1] = B.x[1] - lds.xb[X_X1][hl]; px[2] = B.x[2] - lds.xb[X_X2][hl];
-        pv[0] = B.v[0] - lds.xb[X_V0][hl]; pv[1] = B.v[1] - lds.xb[X_V1][hl]; pv[2] = B.v[2] - lds.xb[X_V2][hl];
+    if (orbiting) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            px[q] = B.x[q] - lds.xb[q][hl];
+            pv[q] = B.v[q] - lds.xb[3 + q][hl];
+        }
         d = sqrt(p2(px[0]) + p2(px[1]) + p2(px[2]));
-        vr = (px[0] * pv[0] + px[1] * pv[1] + px[2] * pv[2]) / d;
         nv2 = p2(pv[0]) + p2(pv[1]) + p2(pv[2]);
         nv = sqrt(nv2);
-        // tides/common.rs:143-170 and rotational_flattening/common.rs:93-120:
-        // scalar products with the spins of the previous evaluation
-        rs_star = px[0] * lds.xb[X_S0][hl] + px[1] * lds.xb[X_S1][hl] + px[2] * lds.xb[X_S2][hl];
-        rs_planet = px[0] * B.s[0] + px[1] * B.s[1] + px[2] * B.s[2];
+        // Universe::initialize runs before the spins are refreshed
+        // (universe.rs:429-430), so the scalar products of tides/common.rs:143-170
+        // and rotational_flattening/common.rs:93-120 use the spins of the
+        // previous evaluation.
+        rs_star = px[0] * lds.s[0][hl] + px[1] * lds.s[1][hl] + px[2] * lds.s[2][hl];
+        rs_planet = px[0] * lds.s[0][lane] + px[1] * lds.s[1][lane] + px[2] * lds.s[2][lane];
     }
+    const Rcp Rd = rcp(d);
+    const double vr = orbiting ? divr(px[0] * pv[0] + px[1] * pv[1] + px[2] * pv[2], Rd) : 0.;
     wsync();
     // ---- evolution and spin = L / I (universe.rs:305-316) ----
-    if (evolution && B.evolution) evolve_body(E, B, B.t);
-    own_spin(B);
-    // ---- round 2: fresh host spin, radius, inertia, angular momentum ----
-    if (B.valid) {
-        lds.xb[X_FS0][lane] = B.s[0]; lds.xb[X_FS1][lane] = B.s[1]; lds.xb[X_FS2][lane] = B.s[2];
-        lds.xb[X_FS2N][lane] = B.s2; lds.xb[X_R][lane] = B.R; lds.xb[X_I][lane] = B.I;
-    }
-    wsync();
-    double hs[3], hs2 = 0., hR = 0., hI = 0.;
-    hs[0] = lds.xb[X_FS0][hl]; hs[1] = lds.xb[X_FS1][hl]; hs[2] = lds.xb[X_FS2][hl];
-    hs2 = lds.xb[X_FS2N][hl]; hR = lds.xb[X_R][hl]; hI = lds.xb[X_I][hl];
+    if (evolution && (B.flags & F_EVOLUTION)) evolve_body(E, lds, B, B.t);
+    own_spin(lds, B);
     wsync();
 
-    double td_acc[3] = {0., 0., 0.}, td_dL[3] = {0., 0., 0.};
-    double rf_acc[3] = {0., 0., 0.}, rf_dL[3] = {0., 0., 0.};
-    double gr_acc[3] = {0., 0., 0.}, gr_dL[3] = {0., 0., 0.};
-    double wind_dL[3] = {0., 0., 0.};
-    double c[C_COUNT];
 #pragma unroll
-    for (int q = 0; q < C_COUNT; ++q) c[q] = 0.;
+    for (int q = 0; q < 3; ++q) { a_add[q] = 0.; dLdt[q] = 0.; }
 
-    // ---- wind (wind.rs:61-91), own body ----
-    if (B.valid && B.wind && B.spin_on && (B.roles & POS_ROLE_WIND)) {
-        const double threshold = sqrt(B.s2);
-        const double shape = sqrt(B.R / R_SUN * 1. / B.m);
-        if (threshold >= B.wsat) {
-            wind_dL[0] = -1. * B.wk * B.s[0] * B.wsat2 * shape;
-            wind_dL[1] = -1. * B.wk * B.s[1] * B.wsat2 * shape;
-            wind_dL[2] = -1. * B.wk * B.s[2] * B.wsat2 * shape;
-        } else {
-            wind_dL[0] = -1. * B.wk * B.s[0] * B.s2 * shape;
-            wind_dL[1] = -1. * B.wk * B.s[1] * B.s2 * shape;
-            wind_dL[2] = -1. * B.wk * B.s[2] * B.s2 * shape;
-        }
-    }
-
-    if (B.valid && !B.is_host) {
-        const double m = B.m, hm = B.hm;
+    if (orbiting) {
+        const double m = B.m, hm = lds.m[hl];
+        const double hs[3] = {lds.s[0][hl], lds.s[1][hl], lds.s[2][hl]};
+        const double s[3] = {lds.s[0][lane], lds.s[1][lane], lds.s[2][lane]};
+        const double hs2 = lds.s2[hl], s2 = lds.s2[lane];
+        const double hR = lds.R[hl], R = lds.R[lane];
+        const double d7 = p7(d);
+        const Rcp Rd7 = rcp(d7);
+        const double d5 = p5(d);
         // ---- tides: orthogonal components (constant_time_lag.rs:206-264) ----
         double os = 0., op = 0.;
-        if (B.tides) {
-            const double d7 = p7(d);
-            os = 4.5 * p2(m) * p10(hR) * B.hsigma / d7;
-            op = 4.5 * p2(hm) * p10(B.R) * B.sigma / d7;
+        if (B.tides()) {
+            os = divr(4.5 * p2(m) * p10(hR) * lds.sigma[hl], Rd7);
+            op = divr(4.5 * p2(hm) * p10(R) * lds.sigma[lane], Rd7);
         }
         // ---- flattening: orthogonal components (oblate_spheroid.rs:21-49) ----
         double fs = 0., fp = 0., ofs = 0., ofp = 0.;
-        if (B.rf) {
-            fs = m * B.hk2f * hs2 * p5(hR) / 6.;
-            ofs = -6. * fs * rs_star / (hs2 * p5(d));
-            fp = hm * B.k2f * B.s2 * p5(B.R) / 6.;
-            ofp = -6. * fp * rs_planet / (B.s2 * p5(d));
+        if (B.rf()) {
+            fs = m * lds.k2f[hl] * hs2 * p5(hR) / 6.;
+            ofs = -6. * fs * rs_star / (hs2 * d5);
+            fp = hm * lds.k2f[lane] * s2 * p5(R) / 6.;
+            ofp = -6. * fp * rs_planet / (s2 * d5);
         }
-        if (B.tides) {
+        if (B.tides()) {
             // radial component (constant_time_lag.rs:266-307)
             const double hm2 = hm * hm;
             const double pm2 = m * m;
-            const double conservative = (-3.0 * K2) / p7(d) * (pm2 * p5(hR) * B.hk2 + hm2 * p5(B.R) * B.k2);
+            const double conservative = divr(-3.0 * K2, Rd7) * (pm2 * p5(hR) * lds.k2[hl] + hm2 * p5(R) * lds.k2[lane]);
             const double factor1 = -13.5 * vr / p8(d);
-            const double term1 = pm2 * p10(hR) * B.hsigma;
-            const double term2 = hm2 * p10(B.R) * B.sigma;
+            const double term1 = pm2 * p10(hR) * lds.sigma[hl];
+            const double term2 = hm2 * p10(R) * lds.sigma[lane];
             const double diss_pm = factor1 * term2;
             const double dissipative = diss_pm + factor1 * term1;
             const double radial = conservative + dissipative;
             // force (constant_time_lag.rs:309-332)
-            const double factor3 = radial + (os + op) * vr / d;
-            double F[3];
-            F[0] = factor3 * px[0] / d + os / d * (hs[1] * px[2] - hs[2] * px[1] - pv[0])
-                 + op / d * (B.s[1] * px[2] - B.s[2] * px[1] - pv[0]);
-            F[1] = factor3 * px[1] / d + os / d * (hs[2] * px[0] - hs[0] * px[2] - pv[1])
-                 + op / d * (B.s[2] * px[0] - B.s[0] * px[2] - pv[1]);
-            F[2] = factor3 * px[2] / d + os / d * (hs[0] * px[1] - hs[1] * px[0] - pv[2])
-                 + op / d * (B.s[0] * px[1] - B.s[1] * px[0] - pv[2]);
-            const double f1m = 1. / m;
-            td_acc[0] = f1m * F[0]; td_acc[1] = f1m * F[1]; td_acc[2] = f1m * F[2];
-            c[C_TF] = F[0]; c[C_TF + 1] = F[1]; c[C_TF + 2] = F[2];
+            const double factor3 = radial + divr((os + op) * vr, Rd);
+            const double od = divr(os, Rd), pd = divr(op, Rd);
+            const double F0 = divr(factor3 * px[0], Rd) + od * (hs[1] * px[2] - hs[2] * px[1] - pv[0])
+                            + pd * (s[1] * px[2] - s[2] * px[1] - pv[0]);
+            const double F1 = divr(factor3 * px[1], Rd) + od * (hs[2] * px[0] - hs[0] * px[2] - pv[1])
+                            + pd * (s[2] * px[0] - s[0] * px[2] - pv[1]);
+            const double F2 = divr(factor3 * px[2], Rd) + od * (hs[0] * px[1] - hs[1] * px[0] - pv[2])
+                            + pd * (s[0] * px[1] - s[1] * px[0] - pv[2]);
+            put3(lds, C_TF, lane, F0, F1, F2);
+            a_add[0] = a_add[0] + B.inv_m * F0;
+            a_add[1] = a_add[1] + B.inv_m * F1;
+            a_add[2] = a_add[2] + B.inv_m * F2;
         }
-        if (B.rf) {
+        if (B.rf()) {
             // radial component (oblate_spheroid.rs:51-60) and force (83-97)
-            const double radial = -3. / p5(d) * (fp + fs)
-                + 15. / p7(d) * (fs * rs_star * rs_star / hs2 + fp * rs_planet * rs_planet / B.s2);
-            double F[3];
-            F[0] = radial * px[0] + ofp * B.s[0] + ofs * hs[0];
-            F[1] = radial * px[1] + ofp * B.s[1] + ofs * hs[1];
-            F[2] = radial * px[2] + ofp * B.s[2] + ofs * hs[2];
-            const double f1m = 1. / m;
-            rf_acc[0] = f1m * F[0]; rf_acc[1] = f1m * F[1]; rf_acc[2] = f1m * F[2];
-            c[C_RF] = F[0]; c[C_RF + 1] = F[1]; c[C_RF + 2] = F[2];
+            const double radial = -3. / d5 * (fp + fs)
+                + divr(15., Rd7) * (fs * rs_star * rs_star / hs2 + fp * rs_planet * rs_planet / s2);
+            const double F0 = radial * px[0] + ofp * s[0] + ofs * hs[0];
+            const double F1 = radial * px[1] + ofp * s[1] + ofs * hs[1];
+            const double F2 = radial * px[2] + ofp * s[2] + ofs * hs[2];
+            put3(lds, C_RF, lane, F0, F1, F2);
+            a_add[0] = a_add[0] + B.inv_m * F0;
+            a_add[1] = a_add[1] + B.inv_m * F1;
+            a_add[2] = a_add[2] + B.inv_m * F2;
         }
-        if (B.spin_on && B.tides) {
-            // torques (constant_time_lag.rs:171-204, tides/common.rs:223-261)
-            const double cx = px[1] * pv[2] - px[2] * pv[1];
-            const double cy = px[2] * pv[0] - px[0] * pv[2];
-            const double cz = px[0] * pv[1] - px[1] * pv[0];
-            double N0 = op * (d * B.s[0] - rs_planet * px[0] / d - 1.0 / d * cx);
-            double N1 = op * (d * B.s[1] - rs_planet * px[1] / d - 1.0 / d * cy);
-            double N2 = op * (d * B.s[2] - rs_planet * px[2] / d - 1.0 / d * cz);
-            td_dL[0] = -1.0 * N0; td_dL[1] = -1.0 * N1; td_dL[2] = -1.0 * N2;
-            N0 = os * (d * hs[0] - rs_star * px[0] / d - 1.0 / d * cx);
-            N1 = os * (d * hs[1] - rs_star * px[1] / d - 1.0 / d * cy);
-            N2 = os * (d * hs[2] - rs_star * px[2] / d - 1.0 / d * cz);
-            c[C_TN] = -1.0 * N0; c[C_TN + 1] = -1.0 * N1; c[C_TN + 2] = -1.0 * N2;
+        if (B.spin_on()) {
+            // torques (constant_time_lag.rs:171-204, tides/common.rs:223-261;
+            // oblate_spheroid.rs:62-81, rotational_flattening/common.rs:165-201)
+            double tdL[3] = {0., 0., 0.}, rdL[3] = {0., 0., 0.};
+            if (B.tides()) {
+                const double cx = px[1] * pv[2] - px[2] * pv[1];
+                const double cy = px[2] * pv[0] - px[0] * pv[2];
+                const double cz = px[0] * pv[1] - px[1] * pv[0];
+                const double id = divr(1.0, Rd);
+                tdL[0] = -1.0 * (op * (d * s[0] - divr(rs_planet * px[0], Rd) - id * cx));
+                tdL[1] = -1.0 * (op * (d * s[1] - divr(rs_planet * px[1], Rd) - id * cy));
+                tdL[2] = -1.0 * (op * (d * s[2] - divr(rs_planet * px[2], Rd) - id * cz));
+                put3(lds, C_TN, lane,
+                     -1.0 * (os * (d * hs[0] - divr(rs_star * px[0], Rd) - id * cx)),
+                     -1.0 * (os * (d * hs[1] - divr(rs_star * px[1], Rd) - id * cy)),
+                     -1.0 * (os * (d * hs[2] - divr(rs_star * px[2], Rd) - id * cz)));
+            }
+            if (B.rf()) {
+                rdL[0] = -1.0 * (ofp * (px[1] * s[2] - px[2] * s[1]));
+                rdL[1] = -1.0 * (ofp * (px[2] * s[0] - px[0] * s[2]));
+                rdL[2] = -1.0 * (ofp * (px[0] * s[1] - px[1] * s[0]));
+                put3(lds, C_RN, lane,
+                     -1.0 * (ofs * (px[1] * hs[2] - px[2] * hs[1])),
+                     -1.0 * (ofs * (px[2] * hs[0] - px[0] * hs[2])),
+                     -1.0 * (ofs * (px[0] * hs[1] - px[1] * hs[0])));
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) dLdt[q] = tdL[q] + rdL[q];
         }
-        if (B.spin_on && B.rf) {
-            // torques (oblate_spheroid.rs:62-81, rotational_flattening/common.rs:165-201)
-            double N0 = ofp * (px[1] * B.s[2] - px[2] * B.s[1]);
-            double N1 = ofp * (px[2] * B.s[0] - px[0] * B.s[2]);
-            double N2 = ofp * (px[0] * B.s[1] - px[1] * B.s[0]);
-            rf_dL[0] = -1.0 * N0; rf_dL[1] = -1.0 * N1; rf_dL[2] = -1.0 * N2;
-            N0 = ofs * (px[1] * hs[2] - px[2] * hs[1]);
-            N1 = ofs * (px[2] * hs[0] - px[0] * hs[2]);
-            N2 = ofs * (px[0] * hs[1] - px[1] * hs[0]);
-            c[C_RN] = -1.0 * N0; c[C_RN + 1] = -1.0 * N1; c[C_RN + 2] = -1.0 * N2;
-        }
-        if (B.gr) {
+        if (B.gr()) {
             // Kidder 1995 (general_relativity.rs:177-456)
-            const double f = B.hmg * B.mg / p2(B.hmg + B.mg);
-            const double mg = B.hmg + B.mg;
+            const Rcp Rhm{hm, lds.inv_m[hl]};
+            const Rcp Rm{m, B.inv_m};
+            const double hmg = lds.mg[hl], pmg = lds.mg[lane];
+            const double f = hmg * pmg / p2(hmg + pmg);
+            const double mg = hmg + pmg;
             const double distance_2 = p2(d);
-            const double mOverM = m / hm;
+            const Rcp Rdc = rcp(distance_2 * SPEED_OF_LIGHT_2);
+            const double mgc = divr(-mg, Rdc);          // -mg / (d^2 c^2)
+            const double mg_d = divr(mg, Rd);
+            const double mOverM = divr(m, Rhm);
+            const Rcp Rnv = rcp(nv);
+            double gr[3];
             {   // 1PN
                 const double radial_velocity_2 = p2(vr);
-                const double radial = -mg / (distance_2 * SPEED_OF_LIGHT_2)
-                    * ((1.0 + 3.0 * f) * nv2 - 2.0 * (2.0 + f) * mg / d - 1.5 * f * radial_velocity_2);
-                const double orthogonal = mg / (distance_2 * SPEED_OF_LIGHT_2) * 2.0 * (2.0 - f) * vr * nv;
+                const double radial = mgc * ((1.0 + 3.0 * f) * nv2 - divr(2.0 * (2.0 + f) * mg, Rd) - 1.5 * f * radial_velocity_2);
+                const double orthogonal = divr(mg, Rdc) * 2.0 * (2.0 - f) * vr * nv;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const double a = radial * px[q] / d + orthogonal * pv[q] / nv;
-                    gr_acc[q] = a;
-                    c[C_G1 + q] = mOverM * a;
-                }
+                for (int q = 0; q < 3; ++q) gr[q] = divr(radial * px[q], Rd) + divr(orthogonal * pv[q], Rnv);
+                put3(lds, C_G1, lane, mOverM * gr[0], mOverM * gr[1], mOverM * gr[2]);
             }
             {   // 2PN
                 const double nv4 = p2(nv2);
                 const double vr2 = p2(vr);
                 const double vr4 = p2(vr2);
                 const double f2 = p2(f);
-                const double radial = -mg / (distance_2 * SPEED_OF_LIGHT_2)
+                const double radial = mgc
                     * (3.0 / 4.0 * (12.0 + 29.0 * f) * (p2(mg) / distance_2)
                        + f * (3.0 - 4.0 * f) * nv4
                        + 15.0 / 8.0 * f * (1.0 - 3.0 * f) * vr4
                        - 3.0 / 2.0 * f * (3.0 - 4.0 * f) * vr2 * nv2
-                       - 0.5 * f * (13.0 - 4.0 * f) * (mg / d) * nv2
-                       - (2.0 + 25.0 * f + 2.0 * f2) * (mg / d) * vr2);
-                const double orthogonal = -mg / (distance_2 * SPEED_OF_LIGHT_2) * (-0.5) * vr
+                       - 0.5 * f * (13.0 - 4.0 * f) * mg_d * nv2
+                       - (2.0 + 25.0 * f + 2.0 * f2) * mg_d * vr2);
+                const double orthogonal = mgc * (-0.5) * vr
                     * (f * (15.0 + 4.0 * f) * nv2
-                       - (4.0 + 41.0 * f + 8.0 * f2) * (mg / d)
+                       - (4.0 + 41.0 * f + 8.0 * f2) * mg_d
                        - 3.0 * f * (3.0 + 2.0 * f) * vr2);
+                double a[3];
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    const double a = radial * px[q] / d + orthogonal * pv[q];
-                    gr_acc[q] = gr_acc[q] + a;
-                    c[C_G2 + q] = mOverM * a;
+                    a[q] = divr(radial * px[q], Rd) + orthogonal * pv[q];
+                    gr[q] = gr[q] + a[q];
                 }
+                put3(lds, C_G2, lane, mOverM * a[0], mOverM * a[1], mOverM * a[2]);
             }
             {   // 1.5PN spin-orbit
+                const double hI = lds.I[hl], I = lds.I[lane];
                 const double Ls[3] = {hI * hs[0], hI * hs[1], hI * hs[2]};
                 const double M = hm + m;
-                const double mass_factor = (hm - m) / M;
-                const double Lp[3] = {B.I * B.s[0], B.I * B.s[1], B.I * B.s[2]};
-                const double n3[3] = {px[0] / d, px[1] / d, px[2] / d};
-                double msf[3];
-#pragma unroll
-                for (int q = 0; q < 3; ++q) msf[q] = mass_factor * M * (Lp[q] / m - Ls[q] / hm);
-                double e1[3];
-                e1[0] = 6. * n3[0] * ((n3[1] * pv[2] - n3[2] * pv[1]) * (2. * (Ls[0] + Lp[0]) + msf[0]));
-                e1[1] = 6. * n3[1] * ((n3[2] * pv[0] - n3[0] * pv[2]) * (2. * (Ls[1] + Lp[1]) + msf[1]));
-                e1[2] = 6. * n3[2] * ((n3[0] * pv[1] - n3[1] * pv[0]) * (2. * (Ls[2] + Lp[2]) + msf[2]));
-                double e7[3], e3s[3];
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    e7[q] = 7. * (Ls[q] + Lp[q]) + 3. * msf[q];
-                    e3s[q] = 3. * (Ls[q] + Lp[q]) + msf[q];
-                }
-                const double e2[3] = {pv[1] * e7[2] - pv[2] * e7[1], pv[2] * e7[0] - pv[0] * e7[2], pv[0] * e7[1] - pv[1] * e7[0]};
-                double e3[3];
-                e3[0] = 3. * vr * (n3[1] * e3s[2] - n3[2] * e3s[1]);
-                e3[1] = 3. * vr * (n3[2] * e3s[0] - n3[0] * e3s[2]);
-                e3[2] = 3. * vr * (n3[0] * e3s[1] - n3[1] * e3s[0]);
+                const Rcp RM = rcp(M);
+                const double mass_factor = divr(hm - m, RM);
+                const double Lp[3] = {I * s[0], I * s[1], I * s[2]};
+                const double n3[3] = {divr(px[0], Rd), divr(px[1], Rd), divr(px[2], Rd)};
                 const double factor_a = G / SPEED_OF_LIGHT_2;
+                double a[3];
+                {
+                    double msf[3], e7[3], e3s[3];
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const double a = factor_a * (e1[q] - e2[q] + e3[q]);
-                    gr_acc[q] = gr_acc[q] + a;
-                    c[C_G3 + q] = mOverM * a;
+                    for (int q = 0; q < 3; ++q) {
+                        msf[q] = mass_factor * M * (divr(Lp[q], Rm) - divr(Ls[q], Rhm));
+                        e7[q] = 7. * (Ls[q] + Lp[q]) + 3. * msf[q];
+                        e3s[q] = 3. * (Ls[q] + Lp[q]) + msf[q];
+                    }
+                    const double e10 = 6. * n3[0] * ((n3[1] * pv[2] - n3[2] * pv[1]) * (2. * (Ls[0] + Lp[0]) + msf[0]));
+                    const double e11 = 6. * n3[1] * ((n3[2] * pv[0] - n3[0] * pv[2]) * (2. * (Ls[1] + Lp[1]) + msf[1]));
+                    const double e12 = 6. * n3[2] * ((n3[0] * pv[1] - n3[1] * pv[0]) * (2. * (Ls[2] + Lp[2]) + msf[2]));
+                    a[0] = factor_a * (e10 - (pv[1] * e7[2] - pv[2] * e7[1]) + 3. * vr * (n3[1] * e3s[2] - n3[2] * e3s[1]));
+                    a[1] = factor_a * (e11 - (pv[2] * e7[0] - pv[0] * e7[2]) + 3. * vr * (n3[2] * e3s[0] - n3[0] * e3s[2]));
+                    a[2] = factor_a * (e12 - (pv[0] * e7[1] - pv[1] * e7[0]) + 3. * vr * (n3[0] * e3s[1] - n3[1] * e3s[0]));
                 }
-                const double mu = (hm * m) / M;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) gr[q] = gr[q] + a[q];
+                put3(lds, C_G3, lane, mOverM * a[0], mOverM * a[1], mOverM * a[2]);
+                const double mu = divr(hm * m, RM);
                 const double Lo[3] = {mu * (px[1] * pv[2] - px[2] * pv[1]), mu * (px[2] * pv[0] - px[0] * pv[2]),
                                       mu * (px[0] * pv[1] - px[1] * pv[0])};
-                double fm = 2. + 3. / 2. * m / hm;
+                double fm = 2. + divr(3. / 2. * m, Rhm);
                 const double sp = n3[0] * Lp[0] + n3[1] * Lp[1] + n3[2] * Lp[2];
-                c[C_GL + 0] = factor_a * (fm * (Lo[1] * Ls[2] - Lo[2] * Ls[1]) - (Lp[1] * Ls[2] - Lp[2] * Ls[1]) + 3. * sp * (n3[1] * Ls[2] - n3[2] * Ls[1]));
-                c[C_GL + 1] = factor_a * (fm * (Lo[2] * Ls[0] - Lo[0] * Ls[2]) - (Lp[2] * Ls[0] - Lp[0] * Ls[2]) + 3. * sp * (n3[2] * Ls[0] - n3[0] * Ls[2]));
-                c[C_GL + 2] = factor_a * (fm * (Lo[0] * Ls[1] - Lo[1] * Ls[0]) - (Lp[0] * Ls[1] - Lp[1] * Ls[0]) + 3. * sp * (n3[0] * Ls[1] - n3[1] * Ls[0]));
-                fm = 2. + 3. / 2. * hm / m;
+                put3(lds, C_GL, lane,
+                     factor_a * (fm * (Lo[1] * Ls[2] - Lo[2] * Ls[1]) - (Lp[1] * Ls[2] - Lp[2] * Ls[1]) + 3. * sp * (n3[1] * Ls[2] - n3[2] * Ls[1])),
+                     factor_a * (fm * (Lo[2] * Ls[0] - Lo[0] * Ls[2]) - (Lp[2] * Ls[0] - Lp[0] * Ls[2]) + 3. * sp * (n3[2] * Ls[0] - n3[0] * Ls[2])),
+                     factor_a * (fm * (Lo[0] * Ls[1] - Lo[1] * Ls[0]) - (Lp[0] * Ls[1] - Lp[1] * Ls[0]) + 3. * sp * (n3[0] * Ls[1] - n3[1] * Ls[0])));
+                fm = 2. + divr(3. / 2. * hm, Rm);
                 const double sh = n3[0] * Ls[0] + n3[1] * Ls[1] + n3[2] * Ls[2];
-                gr_dL[0] = factor_a * (fm * (Lo[1] * Lp[2] - Lo[2] * Lp[1]) - (Ls[1] * Lp[2] - Ls[2] * Lp[1]) + 3. * sh * (n3[1] * Lp[2] - n3[2] * Lp[1]));
-                gr_dL[1] = factor_a * (fm * (Lo[2] * Lp[0] - Lo[0] * Lp[2]) - (Ls[2] * Lp[0] - Ls[0] * Lp[2]) + 3. * sh * (n3[2] * Lp[0] - n3[0] * Lp[2]));
-                gr_dL[2] = factor_a * (fm * (Lo[0] * Lp[1] - Lo[1] * Lp[0]) - (Ls[0] * Lp[1] - Ls[1] * Lp[0]) + 3. * sh * (n3[0] * Lp[1] - n3[1] * Lp[0]));
+                dLdt[0] = dLdt[0] + factor_a * (fm * (Lo[1] * Lp[2] - Lo[2] * Lp[1]) - (Ls[1] * Lp[2] - Ls[2] * Lp[1]) + 3. * sh * (n3[1] * Lp[2] - n3[2] * Lp[1]));
+                dLdt[1] = dLdt[1] + factor_a * (fm * (Lo[2] * Lp[0] - Lo[0] * Lp[2]) - (Ls[2] * Lp[0] - Ls[0] * Lp[2]) + 3. * sh * (n3[2] * Lp[0] - n3[0] * Lp[2]));
+                dLdt[2] = dLdt[2] + factor_a * (fm * (Lo[0] * Lp[1] - Lo[1] * Lp[0]) - (Ls[0] * Lp[1] - Ls[1] * Lp[0]) + 3. * sh * (n3[0] * Lp[1] - n3[1] * Lp[0]));
             }
-        }
-    }
-    // ---- round 3: host reactions, accumulated in body order ----
-    if (B.valid && !B.is_host) {
 #pragma unroll
-        for (int q = 0; q < C_COUNT; ++q) lds.xb[q][lane] = c[q];
-    }
-    wsync();
-    if (B.valid && B.is_host) {
-        double sT[3] = {0., 0., 0.}, sR[3] = {0., 0., 0.}, sTN[3] = {0., 0., 0.}, sRN[3] = {0., 0., 0.};
-        double s1[3] = {0., 0., 0.}, s2[3] = {0., 0., 0.}, s3[3] = {0., 0., 0.}, sGL[3] = {0., 0., 0.};
-        for (int j = 0; j < B.n; ++j) {
-            if (j == B.host) continue;
-            const int l = B.gbase + j;
+            for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + gr[q];
+        } else {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                sT[q] = sT[q] + lds.xb[C_TF + q][l];
-                sR[q] = sR[q] + lds.xb[C_RF + q][l];
-                sTN[q] = sTN[q] + lds.xb[C_TN + q][l];
-                sRN[q] = sRN[q] + lds.xb[C_RN + q][l];
-                s1[q] = s1[q] + lds.xb[C_G1 + q][l];
-                s2[q] = s2[q] + lds.xb[C_G2 + q][l];
-                s3[q] = s3[q] + lds.xb[C_G3 + q][l];
-                sGL[q] = sGL[q] + lds.xb[C_GL + q][l];
-            }
-        }
-        const double factor2 = 1. / B.m;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            if (B.tides) { td_acc[q] = -1.0 * factor2 * sT[q]; if (B.spin_on) td_dL[q] = sTN[q]; }
-            if (B.rf) { rf_acc[q] = -1.0 * factor2 * sR[q]; if (B.spin_on) rf_dL[q] = sRN[q]; }
-            if (B.gr) {
-                gr_acc[q] = -1.0 * s1[q];
-                gr_acc[q] = gr_acc[q] + -1.0 * s2[q];
-                gr_acc[q] = gr_acc[q] + -1.0 * s3[q];
-                gr_dL[q] = sGL[q];
-            }
+            for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + 0.;
         }
     }
     wsync();
-    // ---- outputs (universe.rs:540-614) ----
+    // ---- host reactions, accumulated in body order ----
+    if (B.valid() && B.is_host()) {
+        double s[3];
+        if (B.tides()) {   // tides/common.rs:312-345 and 223-261
+            host_sum3(lds, B, C_TF, s);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        dLdt[q] = td_dL[q] + rf_dL[q] + gr_dL[q] + wind_dL[q];
-        double a = 0.;
-        if (B.tides) a = a + td_acc[q];
-        if (B.rf) a = a + rf_acc[q];
-        if (B.gr) a = a + gr_acc[q];
-        a_add[q] = a;
+            for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + -1.0 * B.inv_m * s[q];
+            if (B.spin_on()) {
+                host_sum3(lds, B, C_TN, s);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) dLdt[q] = s[q];
+            }
+        }
+        if (B.rf()) {      // rotational_flattening/common.rs:203-237 and 165-201
+            host_sum3(lds, B, C_RF, s);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + -1.0 * B.inv_m * s[q];
+            if (B.spin_on()) {
+                host_sum3(lds, B, C_RN, s);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + s[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + 0.;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + 0.;
+        }
+        if (B.gr()) {      // general_relativity.rs:236-238, 295-297, 453-455, 312-315
+            double g[3];
+            host_sum3(lds, B, C_G1, s);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) g[q] = -1.0 * s[q];
+            host_sum3(lds, B, C_G2, s);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) g[q] = g[q] + -1.0 * s[q];
+            host_sum3(lds, B, C_G3, s);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) { g[q] = g[q] + -1.0 * s[q]; a_add[q] = a_add[q] + g[q]; }
+            host_sum3(lds, B, C_GL, s);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + s[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + 0.;
+        }
     }
-}
-
-// Ordered sum over the group of slot `q` (every lane obtains the same value).
-__device__ __forceinline__ double group_sum(const WaveLds &lds, const Lane &B, int q) {
-    double s = 0.;
-    for (int j = 0; j < B.n; ++j) s = s + lds.xb[q][B.gbase + j];
-    return s;
+    wsync();
+    // ---- wind (wind.rs:61-91), own body, added last ----
+    double w[3] = {0., 0., 0.};
+    if (B.valid() && (B.flags & F_WIND) && B.spin_on() && (B.flags & F_WIND_ROLE)) {
+        const double wk = E.wind_k[B.body], wsat = E.wind_sat[B.body];
+        const double s2 = lds.s2[lane];
+        const double threshold = sqrt(s2);
+        const double shape = sqrt(lds.R[lane] / R_SUN * 1. / B.m);
+        const double f = threshold >= wsat ? wsat * wsat : s2;
+        w[0] = -1. * wk * lds.s[0][lane] * f * shape;
+        w[1] = -1. * wk * lds.s[1][lane] * f * shape;
+        w[2] = -1. * wk * lds.s[2][lane] * f * shape;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + w[q];
 }
 
 __device__ inline void integrate_velocity_dependent_forces(const DevEnsemble &E, WaveLds &lds, Lane &B, bool evolution) {
-    const double h = B.half_dt;
+    const double h = 0.5 * B.dt;
+    const bool spin_on = B.spin_on();
     double v_orig[3], L_orig[3], v_fin[3], L_fin[3], dv[3] = {0., 0., 0.}, dL[3] = {0., 0., 0.};
 #pragma unroll
     for (int q = 0; q < 3; ++q) { v_orig[q] = B.v[q]; L_orig[q] = B.L[q]; v_fin[q] = B.v[q]; L_fin[q] = B.L[q]; }
     bool converged = false;
     for (int it = 0; it < MIDPOINT_MAX_ITER; ++it) {
-        double v_prev[3], L_prev[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) { v_prev[q] = v_fin[q]; L_prev[q] = L_fin[q]; }
         double a_add[3], dLdt[3];
         additional_effects(E, lds, B, evolution && it == 0, a_add, dLdt);
+        double pdv = 0., pdL = 0.;   // |change of the final state|^2 of this body
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             dv[q] = h * a_add[q] - B.ev[q];
-            v_fin[q] = v_orig[q] + dv[q];
-            if (B.spin_on) {
+            const double vf = v_orig[q] + dv[q];
+            pdv = pdv + p2(vf - v_fin[q]);
+            v_fin[q] = vf;
+            if (spin_on) {
                 dL[q] = h * dLdt[q] - B.eL[q];
-                L_fin[q] = L_orig[q] + dL[q];
+                const double Lf = L_orig[q] + dL[q];
+                pdL = pdL + p2(Lf - L_fin[q]);
+                L_fin[q] = Lf;
             }
         }
         if (it >= MIDPOINT_MIN_ITER - 1) {
-            // whfast.rs:424-451, sums in body order
-            if (B.valid) {
-                const double dvx = v_fin[0] - v_prev[0], dvy = v_fin[1] - v_prev[1], dvz = v_fin[2] - v_prev[2];
-                lds.xb[0][B.lane] = p2(dvx) + p2(dvy) + p2(dvz);
+            // whfast.rs:424-451, sums over bodies in body order
+            if (B.valid()) {
+                lds.xb[0][B.lane] = pdv;
                 lds.xb[1][B.lane] = p2(v_fin[0]) + p2(v_fin[1]) + p2(v_fin[2]);
-                const double dsx = L_fin[0] - L_prev[0], dsy = L_fin[1] - L_prev[1], dsz = L_fin[2] - L_prev[2];
-                lds.xb[2][B.lane] = p2(dsx) + p2(dsy) + p2(dsz);
+                lds.xb[2][B.lane] = pdL;
                 lds.xb[3][B.lane] = p2(L_fin[0]) + p2(L_fin[1]) + p2(L_fin[2]);
             }
             wsync();
-            const double dv2 = group_sum(lds, B, 0), fv2 = group_sum(lds, B, 1);
-            const double dL2 = group_sum(lds, B, 2), fL2 = group_sum(lds, B, 3);
+            double dv2 = 0., fv2 = 0., dL2 = 0., fL2 = 0.;
+            for (int j = 0; j < B.n; ++j) {
+                const int l = B.gbase + j;
+                dv2 = dv2 + lds.xb[0][l];
+                fv2 = fv2 + lds.xb[1][l];
+                dL2 = dL2 + lds.xb[2][l];
+                fL2 = fL2 + lds.xb[3][l];
+            }
             wsync();
-            const bool ok = (B.spin_on && dv2 / fv2 < DBL_EPS_2 && dL2 / fL2 < DBL_EPS_2) ||
-                            (!B.spin_on && dv2 / fv2 < DBL_EPS_2);
+            const bool ok = (spin_on && dv2 / fv2 < DBL_EPS_2 && dL2 / fL2 < DBL_EPS_2) ||
+                            (!spin_on && dv2 / fv2 < DBL_EPS_2);
             if (ok) { converged = true; break; }
         }
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             B.v[q] = 0.5 * (v_orig[q] + v_fin[q]);
-            if (B.spin_on) B.L[q] = 0.5 * (L_orig[q] + L_fin[q]);
+            if (spin_on) B.L[q] = 0.5 * (L_orig[q] + L_fin[q]);
         }
     }
     if (!converged) B.status |= POS_STATUS_MIDPOINT_NOT_CONVERGED;
@@ -544,7 +605,7 @@ __device__ inline void integrate_velocity_dependent_forces(const DevEnsemble &E,
     for (int q = 0; q < 3; ++q) {
         B.v[q] = v_fin[q];
         B.ev[q] = (B.v[q] - v_orig[q]) - dv[q];
-        if (B.spin_on) {
+        if (spin_on) {
             B.L[q] = L_fin[q];
             B.eL[q] = (B.L[q] - L_orig[q]) - dL[q];
         }
@@ -697,12 +758,34 @@ __device__ inline void kepler_individual_step(Lane &B, double mass_g, double dt)
     }
 }
 
-__device__ inline void kepler_steps(Lane &B, double dt) {
-    if (B.valid && !B.is_host) {
+// Jacobi prefix sums up to body k (whfast.rs:574-576, 642-651): the
+// cumulative G-mass and mass of the host and the bodies before k, k included.
+__device__ inline void jacobi_prefix(const WaveLds &lds, const Lane &B, double *mu, double *eta, int *rank) {
+    const int hl = B.hl();
+    double m_g = lds.mg[hl], e = lds.m[hl];
+    int r = 0;
+    for (int j = 0; j < B.n; ++j) {
+        if (j == B.host) continue;
+        m_g = m_g + lds.mg[B.gbase + j];
+        e = e + lds.m[B.gbase + j];
+        if (j == B.k) break;
+        ++r;
+    }
+    *mu = m_g; *eta = e; *rank = r;
+}
+
+__device__ inline void kepler_steps(const WaveLds &lds, Lane &B, double dt) {
+    if (B.orbiting()) {
         double mu;
-        if (B.coord == POS_COORD_JACOBI) mu = B.jac_mu;
-        else if (B.coord == POS_COORD_DEMOCRATIC_HELIOCENTRIC) mu = B.hmg;
-        else mu = B.hmg + B.mg;
+        if (B.coord == POS_COORD_JACOBI) {
+            double eta;
+            int rank;
+            jacobi_prefix(lds, B, &mu, &eta, &rank);
+        } else if (B.coord == POS_COORD_DEMOCRATIC_HELIOCENTRIC) {
+            mu = lds.mg[B.hl()];
+        } else {
+            mu = lds.mg[B.hl()] + lds.mg[B.lane];
+        }
         kepler_individual_step(B, mu, dt);
     }
     // the host slot (centre of mass) drifts in a straight line (whfast.rs:634-639);
@@ -716,14 +799,14 @@ __device__ inline void kepler_steps(Lane &B, double dt) {
 // ===========================================================================
 __device__ inline void inertial_to_alternative(WaveLds &lds, Lane &B) {
     const int lane = B.lane;
-    if (B.valid) {
+    if (B.valid()) {
         lds.xb[0][lane] = B.x[0]; lds.xb[1][lane] = B.x[1]; lds.xb[2][lane] = B.x[2];
         lds.xb[3][lane] = B.v[0]; lds.xb[4][lane] = B.v[1]; lds.xb[5][lane] = B.v[2];
     }
     wsync();
-    const int hl = B.gbase + B.host;
+    const int hl = B.hl();
     if (B.coord == POS_COORD_JACOBI) {
-        // whfast.rs:889-933: prefix recurrence, recomputed by every lane up to itself
+        // whfast.rs:889-933: prefix recurrence, recomputed by every lane
         double eta = lds.m[hl];
         double s[3], sv[3];
 #pragma unroll
@@ -771,16 +854,18 @@ __device__ inline void inertial_to_alternative(WaveLds &lds, Lane &B) {
             cm = cm + mj;
         }
         B.com_m = cm;
+        const Rcp Rcm = rcp(cm);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) { B.comx[q] = cx[q] / cm; B.comv[q] = cv[q] / cm; }
-        if (B.valid && !B.is_host) {
+        for (int q = 0; q < 3; ++q) { B.comx[q] = divr(cx[q], Rcm); B.comv[q] = divr(cv[q], Rcm); }
+        if (B.orbiting()) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 B.ax[q] = B.x[q] - lds.xb[q][hl];
                 B.av[q] = B.v[q] - B.comv[q];
             }
             if (B.coord == POS_COORD_WHDS) {
-                const double factor = (B.hm + B.m) / B.hm;
+                const double hm = lds.m[hl];
+                const double factor = divr(hm + B.m, Rcp{hm, lds.inv_m[hl]});
 #pragma unroll
                 for (int q = 0; q < 3; ++q) B.av[q] = B.av[q] * factor;
             }
@@ -792,11 +877,13 @@ __device__ inline void inertial_to_alternative(WaveLds &lds, Lane &B) {
 __device__ inline void jump_step(WaveLds &lds, Lane &B, double dt) {
     if (B.coord == POS_COORD_JACOBI) return;
     const int lane = B.lane;
-    if (B.valid && !B.is_host) {
+    if (B.orbiting()) {
         lds.xb[0][lane] = B.av[0]; lds.xb[1][lane] = B.av[1]; lds.xb[2][lane] = B.av[2];
     }
     wsync();
-    const double m0 = B.hm;
+    const int hl = B.hl();
+    const double m0 = lds.m[hl];
+    const Rcp Rm0{m0, lds.inv_m[hl]};
     double p[3] = {0., 0., 0.};
     for (int j = 0; j < B.n; ++j) {
         if (j == B.host) continue;
@@ -806,35 +893,34 @@ __device__ inline void jump_step(WaveLds &lds, Lane &B, double dt) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) p[q] = p[q] + mj * lds.xb[q][l];
         } else {
-            const double factor = m0 + mj;
+            const Rcp Rf = rcp(m0 + mj);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) p[q] = p[q] + mj * lds.xb[q][l] / factor;
+            for (int q = 0; q < 3; ++q) p[q] = p[q] + divr(mj * lds.xb[q][l], Rf);
         }
     }
     wsync();
-    if (B.valid && !B.is_host) {
+    if (B.orbiting()) {
         if (B.coord == POS_COORD_DEMOCRATIC_HELIOCENTRIC) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) B.ax[q] = B.ax[q] + dt * p[q] / m0;
+            for (int q = 0; q < 3; ++q) B.ax[q] = B.ax[q] + divr(dt * p[q], Rm0);
         } else {
-            const double factor = m0 + B.m;
+            const Rcp Rf = rcp(m0 + B.m);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) B.ax[q] = B.ax[q] + dt * (p[q] - (B.m * B.av[q] / factor));
+            for (int q = 0; q < 3; ++q) B.ax[q] = B.ax[q] + dt * (p[q] - divr(B.m * B.av[q], Rf));
         }
     }
 }
 
 __device__ inline void interaction_step(WaveLds &lds, Lane &B, double dt) {
+    const int lane = B.lane;
+    const int hl = B.hl();
     if (B.coord == POS_COORD_JACOBI) {
-        // whfast.rs:935-963 then 566-592
-        const int lane = B.lane;
-        if (B.valid) { lds.xb[0][lane] = B.ag[0]; lds.xb[1][lane] = B.ag[1]; lds.xb[2][lane] = B.ag[2]; }
-        wsync();
-        const int hl = B.gbase + B.host;
+        // whfast.rs:935-963 then 566-592; the gravitational accelerations are
+        // in the persistent LDS area
         double eta = lds.m[hl];
-        double s[3];
+        double s[3], aj_own[3] = {0., 0., 0.};
 #pragma unroll
-        for (int q = 0; q < 3; ++q) s[q] = eta * lds.xb[q][hl];
+        for (int q = 0; q < 3; ++q) s[q] = eta * lds.ag[q][hl];
         for (int j = 0; j < B.n; ++j) {
             if (j == B.host) continue;
             const int l = B.gbase + j;
@@ -844,22 +930,24 @@ __device__ inline void interaction_step(WaveLds &lds, Lane &B, double dt) {
             const double pme = eta * ei;
             double aj[3];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) aj[q] = lds.xb[q][l] - s[q] * ei;
+            for (int q = 0; q < 3; ++q) aj[q] = lds.ag[q][l] - s[q] * ei;
             if (j == B.k) {
 #pragma unroll
-                for (int q = 0; q < 3; ++q) B.aacc[q] = aj[q];
+                for (int q = 0; q < 3; ++q) aj_own[q] = aj[q];
             }
 #pragma unroll
             for (int q = 0; q < 3; ++q) s[q] = s[q] * pme + mj * aj[q];
         }
-        wsync();
-        if (B.valid && !B.is_host) {
+        if (B.orbiting()) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) B.av[q] = B.av[q] + dt * B.aacc[q];
-            if (B.jac_rank > 0) {
+            for (int q = 0; q < 3; ++q) B.av[q] = B.av[q] + dt * aj_own[q];
+            double mu, jeta;
+            int rank;
+            jacobi_prefix(lds, B, &mu, &jeta, &rank);
+            if (rank > 0) {
                 const double rj2i = 1. / (p2(B.ax[0]) + p2(B.ax[1]) + p2(B.ax[2]) + 1e-12);
                 const double rji = sqrt(rj2i);
-                const double rj3im = rji * rj2i * G * B.jac_eta;
+                const double rj3im = rji * rj2i * G * jeta;
                 const double prefac = dt * rj3im;
 #pragma unroll
                 for (int q = 0; q < 3; ++q) B.av[q] = B.av[q] + prefac * B.ax[q];
@@ -867,21 +955,23 @@ __device__ inline void interaction_step(WaveLds &lds, Lane &B, double dt) {
         }
         return;
     }
-    if (B.valid && !B.is_host) {
+    if (B.orbiting()) {
         if (B.coord == POS_COORD_DEMOCRATIC_HELIOCENTRIC) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) B.av[q] = B.av[q] + dt * B.ag[q];
+            for (int q = 0; q < 3; ++q) B.av[q] = B.av[q] + dt * lds.ag[q][lane];
         } else {
-            const double factor = B.hm + B.m;
+            const double hm = lds.m[hl];
+            const double factor = hm + B.m;
+            const Rcp Rhm{hm, lds.inv_m[hl]};
 #pragma unroll
-            for (int q = 0; q < 3; ++q) B.av[q] = B.av[q] + dt * factor * B.ag[q] / B.hm;
+            for (int q = 0; q < 3; ++q) B.av[q] = B.av[q] + divr(dt * factor * lds.ag[q][lane], Rhm);
         }
     }
 }
 
 __device__ inline void alternative_to_inertial(WaveLds &lds, Lane &B) {
     const int lane = B.lane;
-    if (B.valid && !B.is_host) {
+    if (B.orbiting()) {
         lds.xb[0][lane] = B.ax[0]; lds.xb[1][lane] = B.ax[1]; lds.xb[2][lane] = B.ax[2];
         lds.xb[3][lane] = B.av[0]; lds.xb[4][lane] = B.av[1]; lds.xb[5][lane] = B.av[2];
     }
@@ -913,43 +1003,45 @@ __device__ inline void alternative_to_inertial(WaveLds &lds, Lane &B) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) { s[q] = s[q] * eta; sv[q] = sv[q] * eta; }
         }
-        if (B.valid && B.is_host) {
+        if (B.valid() && B.is_host()) {
             const double mtot_i = 1. / eta;
 #pragma unroll
             for (int q = 0; q < 3; ++q) { B.x[q] = s[q] * mtot_i; B.v[q] = sv[q] * mtot_i; }
         }
     } else {
         // whfast.rs:1090-1155
-        const double mtot = B.com_m;
+        const int hl = B.hl();
+        const Rcp Rmt = rcp(B.com_m);
+        const double m0 = lds.m[hl];
+        const Rcp Rm0{m0, lds.inv_m[hl]};
         double nsp[3] = {B.comx[0], B.comx[1], B.comx[2]};
         double nsv[3] = {B.comv[0], B.comv[1], B.comv[2]};
-        const double m0 = B.hm;
         for (int j = 0; j < B.n; ++j) {
             if (j == B.host) continue;
             const int l = B.gbase + j;
             const double mj = lds.m[l];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) nsp[q] = nsp[q] - lds.xb[q][l] * mj / mtot;
+            for (int q = 0; q < 3; ++q) nsp[q] = nsp[q] - divr(lds.xb[q][l] * mj, Rmt);
         }
         for (int j = 0; j < B.n; ++j) {
             if (j == B.host) continue;
             const int l = B.gbase + j;
             const double mj = lds.m[l];
-            const double factor = (B.coord == POS_COORD_WHDS) ? mj / (m0 + mj) : mj / m0;
+            const double factor = (B.coord == POS_COORD_WHDS) ? mj / (m0 + mj) : divr(mj, Rm0);
 #pragma unroll
             for (int q = 0; q < 3; ++q) nsv[q] = nsv[q] - lds.xb[3 + q][l] * factor;
         }
-        if (B.valid) {
-            if (B.is_host) {
+        if (B.valid()) {
+            if (B.is_host()) {
 #pragma unroll
                 for (int q = 0; q < 3; ++q) { B.x[q] = nsp[q]; B.v[q] = nsv[q]; }
             } else {
 #pragma unroll
                 for (int q = 0; q < 3; ++q) B.x[q] = B.ax[q] + nsp[q];
                 if (B.coord == POS_COORD_WHDS) {
-                    const double f = (m0 + B.m) / m0;
+                    const Rcp Rf = rcp(divr(m0 + B.m, Rm0));
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) B.v[q] = B.av[q] / f + B.comv[q];
+                    for (int q = 0; q < 3; ++q) B.v[q] = divr(B.av[q], Rf) + B.comv[q];
                 } else {
 #pragma unroll
                     for (int q = 0; q < 3; ++q) B.v[q] = B.av[q] + B.comv[q];
@@ -965,11 +1057,12 @@ __device__ inline void alternative_to_inertial(WaveLds &lds, Lane &B) {
 // ===========================================================================
 __device__ inline void gravity(const DevEnsemble &E, WaveLds &lds, Lane &B) {
     const int lane = B.lane;
-    if (B.valid) { lds.xb[0][lane] = B.x[0]; lds.xb[1][lane] = B.x[1]; lds.xb[2][lane] = B.x[2]; lds.xb[3][lane] = B.R; }
+    if (B.valid()) { lds.xb[0][lane] = B.x[0]; lds.xb[1][lane] = B.x[1]; lds.xb[2][lane] = B.x[2]; }
     wsync();
-    if (B.valid) {
+    if (B.valid()) {
         const int i = B.k, mm = B.host;
         const bool whfast_one = (B.coord == POS_COORD_JACOBI);
+        const double Ri = lds.R[lane];
         double acc[3] = {0., 0., 0.};
         for (int j = 0; j < B.n; ++j) {
             if (j == i) continue;
@@ -981,7 +1074,7 @@ __device__ inline void gravity(const DevEnsemble &E, WaveLds &lds, Lane &B) {
             if (i < j) {
                 const double rr = E.roche[(size_t)B.body * POS_MAX_BODIES + j];
                 if (distance_2 <= rr * rr) B.status |= POS_STATUS_ROCHE;
-                if (distance_2 <= p2(B.R + lds.xb[3][l])) B.status |= POS_STATUS_COLLISION;
+                if (distance_2 <= p2(Ri + lds.R[l])) B.status |= POS_STATUS_COLLISION;
                 if (i == mm && distance_2 > MAX_DISTANCE_2) B.status |= POS_STATUS_EJECTION;
             }
             if (whfast_one && ((i == mm && mm == 0 && j == 1) || (i == mm && mm > 0 && j == 0) ||
@@ -994,7 +1087,7 @@ __device__ inline void gravity(const DevEnsemble &E, WaveLds &lds, Lane &B) {
             acc[1] = acc[1] + prefact * dy;
             acc[2] = acc[2] + prefact * dz;
         }
-        B.ag[0] = acc[0]; B.ag[1] = acc[1]; B.ag[2] = acc[2];
+        lds.ag[0][lane] = acc[0]; lds.ag[1][lane] = acc[1]; lds.ag[2][lane] = acc[2];
     }
     wsync();
 }
@@ -1003,15 +1096,14 @@ __device__ inline void gravity(const DevEnsemble &E, WaveLds &lds, Lane &B) {
 // OR-ed over the group so that every lane agrees.
 __device__ inline void merge_status(WaveLds &lds, Lane &B) {
     lds.xb[0][B.lane] = __builtin_bit_cast(double, (unsigned long long)B.status);
+    lds.xb[1][B.lane] = (double)B.tw;
     wsync();
     uint32_t st = 0;
     int tw = 0;
     for (int j = 0; j < B.n; ++j) {
         st |= (uint32_t)__builtin_bit_cast(unsigned long long, lds.xb[0][B.gbase + j]);
+        tw = max(tw, (int)lds.xb[1][B.gbase + j]);
     }
-    lds.xb[1][B.lane] = (double)B.tw;
-    wsync();
-    for (int j = 0; j < B.n; ++j) tw = max(tw, (int)lds.xb[1][B.gbase + j]);
     wsync();
     B.status = st;
     B.tw = tw;
@@ -1021,29 +1113,32 @@ __device__ inline void merge_status(WaveLds &lds, Lane &B) {
 // WHFast::iterate (whfast.rs:235-305)
 // ===========================================================================
 __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B) {
-    const bool first = B.last_hist < 0.;
-    const bool trig = B.last_hist + B.hist_period <= B.t;
-    if (first || trig) {
-        if (B.evolution) evolve_body(E, B, B.t);
-        own_spin(B);
-        if (!first) B.last_hist += B.hist_period;
-        else B.last_hist = 0.;
+    {
+        const bool first = B.last_hist < 0.;
+        const double period = E.hist_period[B.sys];
+        if (first || B.last_hist + period <= B.t) {
+            if (B.flags & F_EVOLUTION) evolve_body(E, lds, B, B.t);
+            own_spin(lds, B);
+            wsync();
+            B.last_hist = first ? 0. : B.last_hist + period;
+        }
     }
+    const double half_dt = 0.5 * B.dt;
     integrate_velocity_dependent_forces(E, lds, B, true);
     inertial_to_alternative(lds, B);
-    kepler_steps(B, B.half_dt);
-    jump_step(lds, B, B.half_dt);
+    kepler_steps(lds, B, half_dt);
+    jump_step(lds, B, half_dt);
     alternative_to_inertial(lds, B);
     gravity(E, lds, B);
     interaction_step(lds, B, B.dt);
-    jump_step(lds, B, B.half_dt);
-    kepler_steps(B, B.half_dt);
+    jump_step(lds, B, half_dt);
+    kepler_steps(lds, B, half_dt);
     alternative_to_inertial(lds, B);
     integrate_velocity_dependent_forces(E, lds, B, false);
     B.t = B.t + B.dt;
     B.iteration += 1;
     merge_status(lds, B);
-    if (B.t + B.dt > B.T) B.status |= POS_STATUS_COMPLETED;
+    if (B.t + B.dt > E.time_limit[B.sys]) B.status |= POS_STATUS_COMPLETED;
 }
 
 }  // namespace posdev
