@@ -238,17 +238,22 @@ __device__ __forceinline__ void put3(WaveLds &lds, int slot, int lane, double x,
     lds.xb[slot][lane] = x; lds.xb[slot + 1][lane] = y; lds.xb[slot + 2][lane] = z;
 }
 
-// Ordered sum over the orbiting bodies of the group of a 3-slot contribution.
-__device__ __forceinline__ void host_sum3(const WaveLds &lds, const Lane &B, int slot, double out[3]) {
-    double s0 = 0., s1 = 0., s2 = 0.;
-    for (int j = 0; j < B.n; ++j) {
-        if (j == B.host) continue;
-        const int l = B.gbase + j;
-        s0 = s0 + lds.xb[slot][l];
-        s1 = s1 + lds.xb[slot + 1][l];
-        s2 = s2 + lds.xb[slot + 2][l];
+// Ordered sums over the orbiting bodies of the group of the contribution rows
+// [0, C_COUNT), spread over the group's lanes: lane k sums rows k, k+n, ...
+// Every sum runs over the bodies in index order, so it is exactly the
+// sequential reference sum.  The host's column of the contribution rows is
+// never an input (the host contributes nothing), so each result is written
+// there; after the next wsync() the host reads its reactions in its own column.
+__device__ __forceinline__ void host_reaction_sums(WaveLds &lds, const Lane &B) {
+    const int hl = B.hl();
+    for (int q = B.k; B.valid() && q < C_COUNT; q += B.n) {
+        double acc = 0.;
+        for (int j = 0; j < B.n; ++j) {
+            if (j == B.host) continue;
+            acc = acc + lds.xb[q][B.gbase + j];
+        }
+        lds.xb[q][hl] = acc;
     }
-    out[0] = s0; out[1] = s1; out[2] = s2;
 }
 
 // Accelerations and spin derivatives of every body of the group at the current
@@ -484,48 +489,37 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
     }
     wsync();
     // ---- host reactions, accumulated in body order ----
+    if (B.valid()) host_reaction_sums(lds, B);
+    wsync();
     if (B.valid() && B.is_host()) {
-        double s[3];
+        const double *col = &lds.xb[0][lane];   // this lane's column, row stride WAVE
+        auto row = [&](int q) { return col[q * WAVE]; };
         if (B.tides()) {   // tides/common.rs:312-345 and 223-261
-            host_sum3(lds, B, C_TF, s);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + -1.0 * B.inv_m * s[q];
+            for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + -1.0 * B.inv_m * row(C_TF + q);
             if (B.spin_on()) {
-                host_sum3(lds, B, C_TN, s);
 #pragma unroll
-                for (int q = 0; q < 3; ++q) dLdt[q] = s[q];
+                for (int q = 0; q < 3; ++q) dLdt[q] = row(C_TN + q);
             }
         }
         if (B.rf()) {      // rotational_flattening/common.rs:203-237 and 165-201
-            host_sum3(lds, B, C_RF, s);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + -1.0 * B.inv_m * s[q];
-            if (B.spin_on()) {
-                host_sum3(lds, B, C_RN, s);
+            for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + -1.0 * B.inv_m * row(C_RF + q);
 #pragma unroll
-                for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + s[q];
-            } else {
-#pragma unroll
-                for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + 0.;
-            }
+            for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + (B.spin_on() ? row(C_RN + q) : 0.);
         } else {
 #pragma unroll
             for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + 0.;
         }
         if (B.gr()) {      // general_relativity.rs:236-238, 295-297, 453-455, 312-315
-            double g[3];
-            host_sum3(lds, B, C_G1, s);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) g[q] = -1.0 * s[q];
-            host_sum3(lds, B, C_G2, s);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) g[q] = g[q] + -1.0 * s[q];
-            host_sum3(lds, B, C_G3, s);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) { g[q] = g[q] + -1.0 * s[q]; a_add[q] = a_add[q] + g[q]; }
-            host_sum3(lds, B, C_GL, s);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + s[q];
+            for (int q = 0; q < 3; ++q) {
+                double g = -1.0 * row(C_G1 + q);
+                g = g + -1.0 * row(C_G2 + q);
+                g = g + -1.0 * row(C_G3 + q);
+                a_add[q] = a_add[q] + g;
+                dLdt[q] = dLdt[q] + row(C_GL + q);
+            }
         } else {
 #pragma unroll
             for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + 0.;
@@ -581,14 +575,15 @@ __device__ inline void integrate_velocity_dependent_forces(const DevEnsemble &E,
                 lds.xb[3][B.lane] = p2(L_fin[0]) + p2(L_fin[1]) + p2(L_fin[2]);
             }
             wsync();
-            double dv2 = 0., fv2 = 0., dL2 = 0., fL2 = 0.;
-            for (int j = 0; j < B.n; ++j) {
-                const int l = B.gbase + j;
-                dv2 = dv2 + lds.xb[0][l];
-                fv2 = fv2 + lds.xb[1][l];
-                dL2 = dL2 + lds.xb[2][l];
-                fL2 = fL2 + lds.xb[3][l];
+            // four ordered sums over all bodies, spread over the lanes, into rows 4-7
+            for (int q = B.k; B.valid() && q < 4; q += B.n) {
+                double acc = 0.;
+                for (int j = 0; j < B.n; ++j) acc = acc + lds.xb[q][B.gbase + j];
+                lds.xb[4 + q][B.gbase] = acc;
             }
+            wsync();
+            const double dv2 = lds.xb[4][B.gbase], fv2 = lds.xb[5][B.gbase];
+            const double dL2 = lds.xb[6][B.gbase], fL2 = lds.xb[7][B.gbase];
             wsync();
             const bool ok = (spin_on && dv2 / fv2 < DBL_EPS_2 && dL2 / fL2 < DBL_EPS_2) ||
                             (!spin_on && dv2 / fv2 < DBL_EPS_2);
@@ -839,20 +834,23 @@ __device__ inline void inertial_to_alternative(WaveLds &lds, Lane &B) {
 #pragma unroll
         for (int q = 0; q < 3; ++q) { B.comx[q] = s[q] * mtot_i; B.comv[q] = sv[q] * mtot_i; }
     } else {
-        // whfast.rs:973-1023: centre of mass, host first then the others in order
-        double cx[3] = {0., 0., 0.}, cv[3] = {0., 0., 0.}, cm = 0.;
-        for (int jj = -1; jj < B.n; ++jj) {
-            const int j = jj < 0 ? B.host : jj;
-            if (jj >= 0 && jj == B.host) continue;
-            const int l = B.gbase + j;
-            const double mj = lds.m[l];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                cx[q] = cx[q] + lds.xb[q][l] * mj;
-                cv[q] = cv[q] + lds.xb[3 + q][l] * mj;
+        // whfast.rs:973-1023: centre of mass, host first then the others in
+        // order; the seven ordered sums are spread over the group's lanes and
+        // left in rows 6-12 of column gbase
+        for (int q = B.k; B.valid() && q < 7; q += B.n) {
+            double acc = 0.;
+            for (int jj = -1; jj < B.n; ++jj) {
+                const int j = jj < 0 ? B.host : jj;
+                if (jj >= 0 && jj == B.host) continue;
+                const int l = B.gbase + j;
+                acc = acc + (q < 6 ? lds.xb[q][l] * lds.m[l] : lds.m[l]);
             }
-            cm = cm + mj;
+            lds.xb[6 + q][B.gbase] = acc;
         }
+        wsync();
+        const double cx[3] = {lds.xb[6][B.gbase], lds.xb[7][B.gbase], lds.xb[8][B.gbase]};
+        const double cv[3] = {lds.xb[9][B.gbase], lds.xb[10][B.gbase], lds.xb[11][B.gbase]};
+        const double cm = lds.xb[12][B.gbase];
         B.com_m = cm;
         const Rcp Rcm = rcp(cm);
 #pragma unroll
@@ -884,20 +882,21 @@ __device__ inline void jump_step(WaveLds &lds, Lane &B, double dt) {
     const int hl = B.hl();
     const double m0 = lds.m[hl];
     const Rcp Rm0{m0, lds.inv_m[hl]};
-    double p[3] = {0., 0., 0.};
-    for (int j = 0; j < B.n; ++j) {
-        if (j == B.host) continue;
-        const int l = B.gbase + j;
-        const double mj = lds.m[l];
-        if (B.coord == POS_COORD_DEMOCRATIC_HELIOCENTRIC) {
-#pragma unroll
-            for (int q = 0; q < 3; ++q) p[q] = p[q] + mj * lds.xb[q][l];
-        } else {
-            const Rcp Rf = rcp(m0 + mj);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) p[q] = p[q] + divr(mj * lds.xb[q][l], Rf);
+    // three ordered momentum sums, spread over the lanes (valid lanes only:
+    // an empty group has n = 0), left in the host's column
+    for (int q = B.k; B.valid() && q < 3; q += B.n) {
+        double acc = 0.;
+        for (int j = 0; j < B.n; ++j) {
+            if (j == B.host) continue;
+            const int l = B.gbase + j;
+            const double mj = lds.m[l];
+            if (B.coord == POS_COORD_DEMOCRATIC_HELIOCENTRIC) acc = acc + mj * lds.xb[q][l];
+            else acc = acc + divr(mj * lds.xb[q][l], rcp(m0 + mj));
         }
+        lds.xb[q][hl] = acc;
     }
+    wsync();
+    const double p[3] = {lds.xb[0][hl], lds.xb[1][hl], lds.xb[2][hl]};
     wsync();
     if (B.orbiting()) {
         if (B.coord == POS_COORD_DEMOCRATIC_HELIOCENTRIC) {
@@ -1014,23 +1013,28 @@ __device__ inline void alternative_to_inertial(WaveLds &lds, Lane &B) {
         const Rcp Rmt = rcp(B.com_m);
         const double m0 = lds.m[hl];
         const Rcp Rm0{m0, lds.inv_m[hl]};
-        double nsp[3] = {B.comx[0], B.comx[1], B.comx[2]};
-        double nsv[3] = {B.comv[0], B.comv[1], B.comv[2]};
-        for (int j = 0; j < B.n; ++j) {
-            if (j == B.host) continue;
-            const int l = B.gbase + j;
-            const double mj = lds.m[l];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) nsp[q] = nsp[q] - divr(lds.xb[q][l] * mj, Rmt);
+        // six ordered chains (host position and velocity), one per lane,
+        // left in the host's column
+        for (int q = B.k; B.valid() && q < 6; q += B.n) {
+            // constant indices only: a dynamic index would move comx/comv to the stack
+            double acc = q == 0 ? B.comx[0] : q == 1 ? B.comx[1] : q == 2 ? B.comx[2]
+                       : q == 3 ? B.comv[0] : q == 4 ? B.comv[1] : B.comv[2];
+            for (int j = 0; j < B.n; ++j) {
+                if (j == B.host) continue;
+                const int l = B.gbase + j;
+                const double mj = lds.m[l];
+                if (q < 3) {
+                    acc = acc - divr(lds.xb[q][l] * mj, Rmt);
+                } else {
+                    const double factor = (B.coord == POS_COORD_WHDS) ? mj / (m0 + mj) : divr(mj, Rm0);
+                    acc = acc - lds.xb[q][l] * factor;
+                }
+            }
+            lds.xb[q][hl] = acc;
         }
-        for (int j = 0; j < B.n; ++j) {
-            if (j == B.host) continue;
-            const int l = B.gbase + j;
-            const double mj = lds.m[l];
-            const double factor = (B.coord == POS_COORD_WHDS) ? mj / (m0 + mj) : divr(mj, Rm0);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) nsv[q] = nsv[q] - lds.xb[3 + q][l] * factor;
-        }
+        wsync();
+        const double nsp[3] = {lds.xb[0][hl], lds.xb[1][hl], lds.xb[2][hl]};
+        const double nsv[3] = {lds.xb[3][hl], lds.xb[4][hl], lds.xb[5][hl]};
         if (B.valid()) {
             if (B.is_host()) {
 #pragma unroll
