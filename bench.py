@@ -73,6 +73,23 @@ def timed_steps(ens, steps, events):
     return wall, events.elapsed_ms()
 
 
+def measured_traffic(workload, steps_per_launch):
+    """HBM bytes per launch of the step kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of the same workload and launch shape
+    (profiles/*.md, `traffic_json:` lines, counters already scaled by the
+    calibration copy); None when no summary matches."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_json:"):
+                    rec = json.loads(line.split(":", 1)[1])
+                    if rec["workload"] == workload and rec["steps_per_launch"] == steps_per_launch:
+                        rec["source"] = os.path.relpath(path, ROOT)
+                        return rec
+    return None
+
+
 def cpu_baseline(spec, threads, target_s=12.0):
     """The oracle (C++ restatement of the reference step) on the host cores:
     one system per thread at a time, `threads` threads, a bounded sample."""
@@ -132,7 +149,7 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    from posidonius_amd import generators as gen
+    from posidonius_amd import abi, generators as gen
     from posidonius_amd.ensemble import Ensemble
 
     defaults = {1: 4096, 2: 65536, 3: 16384, 4: 1024}
@@ -158,8 +175,11 @@ def main():
     ens.synchronize()
     log(f"timing {args.steps} steps")
     wall, ev_ms = timed_steps(ens, args.steps, events)
-    st = ens.get_state(fields=("status",))
-    frozen = int(np.count_nonzero(st["status"] & 0x11e))
+    t0 = time.perf_counter()
+    full_state = ens.get_state()     # every field, device -> host
+    readback_s = time.perf_counter() - t0
+    frozen = int(np.count_nonzero(full_state["status"] & abi.STATUS_FATAL))
+    del full_state
     ens.close()
     launches = math.ceil(args.steps / K)
     value = n_sys * args.steps / wall
@@ -192,6 +212,8 @@ def main():
         log(f"cpu: {cpu['value']:.4g} system-steps/s")
 
     mean_n = float(counts.mean())
+    workload_key = {1: "uniform", 2: "uneven", 3: "binary", 4: "adjoint"}[args.config] + str(n_sys)
+    traffic = measured_traffic(workload_key, K) if args.steps >= K else None
     result = {
         "metric": "ensemble system-steps/s FP64 full effects (batched vs plain GPU); adjoint/fwd cost",
         "value": value,
@@ -223,7 +245,11 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "traffic_source": (f"rocprofv3 FETCH_SIZE x{traffic.get('fetch_scale', 2.0)} + WRITE_SIZE x"
+                               f"{traffic.get('write_scale', 1.0)} per launch ({traffic['source']})") if traffic else None,
+            "achieved_hbm_gbs": (traffic["bytes_per_launch"] / (kernel_ms * 1e-3) / 1e9) if traffic else None,
+            "hbm_peak_gbs": HBM_PEAK_GBS,
             "kernel": "whfast_steps_kernel",
             "kernel_ms_per_launch": kernel_ms,
             "flops_per_launch": flops_per_launch,
@@ -233,6 +259,9 @@ def main():
         "plain_gpu": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
         "frozen_systems": frozen,
+        "pcie_inclusive": {"value": n_sys * args.steps / (wall + readback_s), "unit": "system-steps/s",
+                           "readback_ms": readback_s * 1e3,
+                           "note": "the timed steps plus one pos_get_state of every field to host memory"},
     }
     print(json.dumps(result), flush=True)
 

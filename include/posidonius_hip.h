@@ -7,7 +7,7 @@
  * single-system; this ABI is the same contract lifted to an ensemble of
  * independent systems that live in device memory behind an opaque handle.
  * A future Rust `impl Integrator for WHFastHip` binds these symbols one to
- * one (see INTEGRATION.md for the cgo/ctypes/Rust-FFI stubs).
+ * one.
  *
  * Conventions
  *   - Every function returns 0 on success and a negative POS_E_* code on

@@ -21,7 +21,7 @@
 // square-and-multiply chain; pw2..pw10 below reproduce those chains.
 // Compile with -ffp-contract=off: the reference never fuses multiply-adds.
 //
-// Deliberate departures from the reference (documented in DESIGN.md):
+// Deliberate departures from the reference:
 //   * panic!() paths set status bits and freeze the system (the step in which
 //     they fire still completes);
 //   * loops the reference leaves unbounded (Stumpff argument reduction on an

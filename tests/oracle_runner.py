@@ -33,6 +33,7 @@ class Oracle:
         self.lib.oracle_run.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         self.lib.oracle_gravity.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         self.lib.oracle_additional_effects.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        self.lib.oracle_flops.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
 
     def _err(self):
         return self.lib.oracle_last_error().decode()
@@ -49,6 +50,16 @@ class Oracle:
         arr["steps_done"] = steps
         arr["midpoint_evaluations"] = mid
         return arr
+
+    def flops(self, spec, n_steps):
+        """FP64 operations (+ - * / sqrt, each counted once) of n_steps
+        iterations of every system, counted by the oracle's FlopCount scalar."""
+        h = spec.desc()
+        out = np.zeros(spec.n_systems)
+        steps = np.zeros(spec.n_systems, dtype=np.int64)
+        if self.lib.oracle_flops(C.addressof(h.desc), int(n_steps), out.ctypes.data, steps.ctypes.data):
+            raise RuntimeError(self._err())
+        return out, steps
 
     def gravity(self, spec):
         h = spec.desc()
