@@ -164,3 +164,85 @@ int oracle_flops(const pos_ensemble_desc *d, int64_t n_steps, double *out_flops,
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Tangent-linear (variational) integration: the Dual<ORACLE_TANGENT_DIRS>
+// instantiation of the restated step.  Direction k seeds the initial
+// position, velocity and spin angular momentum with d_x/d_v/d_L[k] ([3*nb]
+// each, may be NULL = 0) and the parameters sigma (scale * dissipation
+// factor), tidal k2 and fluid k2 with d_sigma/d_k2/d_k2f[k] ([nb], may be
+// NULL).  out_* receive, per direction, the tangents of the final position,
+// velocity and angular momentum ([n_dirs][3*nb]); *out (optional) the final
+// values.  Runs every system for n_steps steps on n_threads threads.
+constexpr int ORACLE_TANGENT_DIRS = 8;
+int oracle_tangent_dirs(void) { return ORACLE_TANGENT_DIRS; }
+
+int oracle_tangent(const pos_ensemble_desc *d, int64_t n_steps, int32_t n_threads, int32_t n_dirs,
+                   const double *d_x, const double *d_v, const double *d_L, const double *d_sigma,
+                   const double *d_k2, const double *d_k2f, pos_state *out, double *out_x, double *out_v,
+                   double *out_L) {
+    using R = oracle::Dual<ORACLE_TANGENT_DIRS>;
+    if (n_dirs < 1 || n_dirs > ORACLE_TANGENT_DIRS) { g_err = "n_dirs out of range"; return -1; }
+    const int ns = d->n_systems;
+    const int nb = d->body_offset[ns];
+    std::atomic<int> next{0};
+    std::atomic<int> failed{-1};
+    std::string first_err;
+    auto worker = [&]() {
+        oracle::System<R> *S = new oracle::System<R>();
+        for (;;) {
+            const int s = next.fetch_add(1);
+            if (s >= ns) break;
+            std::string err;
+            if (!oracle::load_system(d, s, *S, &err)) {
+                int expected = -1;
+                if (failed.compare_exchange_strong(expected, s)) first_err = err;
+                continue;
+            }
+            const int b0 = d->body_offset[s];
+            for (int i = 0; i < S->n; ++i) {
+                const int b = b0 + i;
+                auto &P = S->p[i];
+                for (int k = 0; k < n_dirs; ++k) {
+                    const size_t o3 = (size_t)k * 3 * nb + 3 * b, o1 = (size_t)k * nb + b;
+                    if (d_x) { P.x.x.d[k] = d_x[o3]; P.x.y.d[k] = d_x[o3 + 1]; P.x.z.d[k] = d_x[o3 + 2]; }
+                    if (d_v) { P.v.x.d[k] = d_v[o3]; P.v.y.d[k] = d_v[o3 + 1]; P.v.z.d[k] = d_v[o3 + 2]; }
+                    if (d_L) { P.L.x.d[k] = d_L[o3]; P.L.y.d[k] = d_L[o3 + 1]; P.L.z.d[k] = d_L[o3 + 2]; }
+                    if (d_sigma) P.sigma.d[k] = d_sigma[o1];
+                    if (d_k2) P.k2.d[k] = d_k2[o1];
+                    if (d_k2f) P.k2f.d[k] = d_k2f[o1];
+                }
+            }
+            if (oracle::val(S->t) == 0.) oracle::initialize_physical_values(*S);
+            int64_t k = 0;
+            while (k < n_steps) {
+                ++k;
+                if (!oracle::iterate(*S)) break;
+            }
+            if (out) store_state(*S, b0, s, out);
+            for (int i = 0; i < S->n; ++i) {
+                const int b = b0 + i;
+                const auto &P = S->p[i];
+                for (int q = 0; q < n_dirs; ++q) {
+                    const size_t o3 = (size_t)q * 3 * nb + 3 * b;
+                    if (out_x) { out_x[o3] = P.x.x.d[q]; out_x[o3 + 1] = P.x.y.d[q]; out_x[o3 + 2] = P.x.z.d[q]; }
+                    if (out_v) { out_v[o3] = P.v.x.d[q]; out_v[o3 + 1] = P.v.y.d[q]; out_v[o3 + 2] = P.v.z.d[q]; }
+                    if (out_L) { out_L[o3] = P.L.x.d[q]; out_L[o3 + 1] = P.L.y.d[q]; out_L[o3 + 2] = P.L.z.d[q]; }
+                }
+            }
+        }
+        delete S;
+    };
+    const int nt = n_threads < 1 ? 1 : n_threads;
+    std::vector<std::thread> pool;
+    for (int i = 0; i < nt; ++i) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
+    if (failed.load() >= 0) {
+        g_err = "system " + std::to_string(failed.load()) + ": " + first_err;
+        return -1;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -33,3 +33,55 @@ inline FlopCount sqrt_r(const FlopCount &x) { ++FlopCount::count; return FlopCou
 inline FlopCount abs_r(const FlopCount &x) { return FlopCount(std::fabs(x.v)); }
 
 }  // namespace oracle
+
+namespace oracle {
+
+// Dual<K>: a binary64 value with K tangent directions (forward-mode
+// differentiation).  Running the restated step with it is the tangent-linear
+// (variational) integration of the discrete step: the derivative of exactly
+// the operations executed, including the iterations of the Kepler solver and
+// of the implicit midpoint.  Comparisons look at the value only.
+template <int K> struct Dual {
+    double v = 0.;
+    double d[K] = {};
+    Dual() = default;
+    Dual(double x) : v(x) {}
+};
+template <int K> inline Dual<K> operator+(const Dual<K> &a, const Dual<K> &b) {
+    Dual<K> r(a.v + b.v);
+    for (int k = 0; k < K; ++k) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+template <int K> inline Dual<K> operator-(const Dual<K> &a, const Dual<K> &b) {
+    Dual<K> r(a.v - b.v);
+    for (int k = 0; k < K; ++k) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+template <int K> inline Dual<K> operator*(const Dual<K> &a, const Dual<K> &b) {
+    Dual<K> r(a.v * b.v);
+    for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+template <int K> inline Dual<K> operator/(const Dual<K> &a, const Dual<K> &b) {
+    Dual<K> r(a.v / b.v);
+    const double ib = 1. / b.v;
+    for (int k = 0; k < K; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
+    return r;
+}
+template <int K> inline Dual<K> operator-(const Dual<K> &a) {
+    Dual<K> r(-a.v);
+    for (int k = 0; k < K; ++k) r.d[k] = -a.d[k];
+    return r;
+}
+template <int K> inline bool operator==(const Dual<K> &a, const Dual<K> &b) { return a.v == b.v; }
+template <int K> inline bool operator!=(const Dual<K> &a, const Dual<K> &b) { return a.v != b.v; }
+template <int K> inline double val(const Dual<K> &x) { return x.v; }
+template <int K> inline Dual<K> sqrt_r(const Dual<K> &x) {
+    Dual<K> r(std::sqrt(x.v));
+    const double h = 0.5 / r.v;
+    for (int k = 0; k < K; ++k) r.d[k] = x.d[k] * h;
+    return r;
+}
+template <int K> inline Dual<K> abs_r(const Dual<K> &x) { return x.v < 0. ? -x : x; }
+
+}  // namespace oracle

@@ -34,6 +34,7 @@ class Oracle:
         self.lib.oracle_gravity.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         self.lib.oracle_additional_effects.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         self.lib.oracle_flops.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+        self.lib.oracle_tangent.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32] + [C.c_void_p] * 10
 
     def _err(self):
         return self.lib.oracle_last_error().decode()
@@ -60,6 +61,28 @@ class Oracle:
         if self.lib.oracle_flops(C.addressof(h.desc), int(n_steps), out.ctypes.data, steps.ctypes.data):
             raise RuntimeError(self._err())
         return out, steps
+
+    def tangent(self, spec, n_steps, dirs, threads=8):
+        """Tangent-linear run: `dirs` maps any of x, v, L ([K, nb, 3]) and
+        sigma, k2, k2f ([K, nb]) to K seed directions (K <= 8).  Returns the
+        final state and the tangents of x, v, L, each [K, nb, 3]."""
+        K = next(iter(dirs.values())).shape[0]
+        nb = spec.n_bodies
+        h = spec.desc()
+        st, arr = abi.alloc_state(spec.n_systems, nb)
+        seeds = {}
+        for k in ("x", "v", "L", "sigma", "k2", "k2f"):
+            a = dirs.get(k)
+            seeds[k] = None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        outs = {k: np.zeros((K, nb, 3)) for k in ("x", "v", "L")}
+        ptr = lambda a: None if a is None else a.ctypes.data
+        rc = self.lib.oracle_tangent(C.addressof(h.desc), int(n_steps), int(threads), int(K),
+                                     ptr(seeds["x"]), ptr(seeds["v"]), ptr(seeds["L"]), ptr(seeds["sigma"]),
+                                     ptr(seeds["k2"]), ptr(seeds["k2f"]), C.addressof(st),
+                                     outs["x"].ctypes.data, outs["v"].ctypes.data, outs["L"].ctypes.data)
+        if rc != 0:
+            raise RuntimeError(self._err())
+        return arr, outs
 
     def gravity(self, spec):
         h = spec.desc()
