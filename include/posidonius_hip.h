@@ -259,6 +259,42 @@ int32_t pos_tune(pos_ens *ens, int32_t probe_steps, pos_options *chosen);
 int32_t pos_gravity_acceleration(pos_ens *ens, double *out_acc, uint32_t *out_status);
 int32_t pos_additional_effects(pos_ens *ens, double *out_acc, double *out_dangular_momentum_dt);
 
+/*
+ * Discrete adjoint (the differentiable form).  Supported systems: democratic
+ * heliocentric coordinates, any of tides / rotational flattening / Kidder 1995
+ * GR, no wind, disk or evolution tables.  No counterpart in the reference,
+ * which has no derivatives; it differentiates the step of whfast.rs:235-305.
+ *
+ * pos_adjoint_record advances every system by n_steps like pos_step and
+ * keeps a checkpoint of the state every segment_steps steps (0 = 1000).
+ * pos_adjoint then returns the vector-Jacobian product of that run: from
+ * the cotangents of the final position, velocity and spin angular momentum
+ * (seed), the cotangents of the state at the start of the recording and the
+ * gradients with respect to every body's CTL dissipation factor sigma
+ * (dissipation_factor_scale * dissipation_factor), tidal Love number and
+ * fluid Love number (grad).  It recomputes each segment from its checkpoint,
+ * runs the reverse sweep over it, and leaves the ensemble's state where
+ * pos_adjoint_record left it.  Arrays are in input body order, [3*nb] for
+ * vectors and [nb] for parameters; NULL seed arrays count as zero and NULL
+ * grad arrays are skipped.  Systems that stop early (completion or a fatal
+ * status) are differentiated over the steps they executed.
+ */
+typedef struct pos_adjoint_seed {
+    const double *position;
+    const double *velocity;
+    const double *angular_momentum;
+} pos_adjoint_seed;
+typedef struct pos_adjoint_grad {
+    double *position;
+    double *velocity;
+    double *angular_momentum;
+    double *dissipation_factor;   /* d/d sigma, sigma = scale * factor */
+    double *love_number;          /* d/d tidal k2                      */
+    double *fluid_love_number;    /* d/d flattening k2                 */
+} pos_adjoint_grad;
+int32_t pos_adjoint_record(pos_ens *ens, int64_t n_steps, int32_t segment_steps);
+int32_t pos_adjoint(pos_ens *ens, const pos_adjoint_seed *seed, pos_adjoint_grad *grad);
+
 /* Release device memory and the stream. */
 void pos_destroy(pos_ens *ens);
 

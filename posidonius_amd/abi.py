@@ -150,6 +150,15 @@ class Options(C.Structure):
     ]
 
 
+class AdjointSeed(C.Structure):
+    _fields_ = [("position", _pd), ("velocity", _pd), ("angular_momentum", _pd)]
+
+
+class AdjointGrad(C.Structure):
+    _fields_ = [("position", _pd), ("velocity", _pd), ("angular_momentum", _pd),
+                ("dissipation_factor", _pd), ("love_number", _pd), ("fluid_love_number", _pd)]
+
+
 def ptr(a, ctype):
     """Pointer to the data of a contiguous numpy array, or NULL for None."""
     if a is None:
@@ -254,6 +263,10 @@ def _declare(lib):
     lib.pos_gravity_acceleration.restype = C.c_int32
     lib.pos_additional_effects.argtypes = [C.c_void_p, _pd, _pd]
     lib.pos_additional_effects.restype = C.c_int32
+    lib.pos_adjoint_record.argtypes = [C.c_void_p, C.c_int64, C.c_int32]
+    lib.pos_adjoint_record.restype = C.c_int32
+    lib.pos_adjoint.argtypes = [C.c_void_p, C.POINTER(AdjointSeed), C.POINTER(AdjointGrad)]
+    lib.pos_adjoint.restype = C.c_int32
     lib.pos_destroy.argtypes = [C.c_void_p]
     lib.pos_destroy.restype = None
 
@@ -262,7 +275,7 @@ EXPORTED_SYMBOLS = (
     "pos_abi_version", "pos_last_error", "pos_create", "pos_step", "pos_synchronize",
     "pos_get_state", "pos_set_state", "pos_set_time_limit", "pos_set_historic_snapshot_period",
     "pos_get_counts", "pos_get_stream", "pos_get_options", "pos_tune", "pos_gravity_acceleration",
-    "pos_additional_effects", "pos_destroy",
+    "pos_additional_effects", "pos_adjoint_record", "pos_adjoint", "pos_destroy",
 )
 
 

@@ -56,6 +56,9 @@ class EnsembleSpec:
     evolver_offset: Optional[np.ndarray] = None
     tables: Dict[str, np.ndarray] = field(default_factory=dict)
     names: List[str] = field(default_factory=list)
+    # optional per-body osculating elements the initial state was built from
+    # (host rows zero): a, e, inc, varpi (longitude of perihelion), node, mean_anomaly
+    elements: Dict[str, np.ndarray] = field(default_factory=dict)
 
     @property
     def n_systems(self) -> int:
@@ -103,6 +106,8 @@ class EnsembleSpec:
             out.evolver_offset = np.asarray(eo, dtype=np.int32)
             out.tables = {k: (np.concatenate(v) if v else np.zeros(0)).astype(np.float64) for k, v in tabs.items()}
         out.names = sum((s.names or [""] * s.n_systems for s in specs), [])
+        if specs and all(s.elements for s in specs):
+            out.elements = {k: np.concatenate([s.elements[k] for s in specs]) for k in specs[0].elements}
         return out
 
     def subset(self, systems) -> "EnsembleSpec":
@@ -120,6 +125,7 @@ class EnsembleSpec:
             out.evolver_offset = (self.evolver_offset[b0:b1 + 1] - e0).astype(np.int32)
             out.tables = {k: v[e0:e1].copy() for k, v in self.tables.items()}
         out.names = [self.names[s]] if self.names else []
+        out.elements = {k: v[b0:b1].copy() for k, v in self.elements.items()}
         return out
 
     def copy(self) -> "EnsembleSpec":

@@ -1,0 +1,415 @@
+// adjoint_kernels.hpp -- reverse sweep of the discrete adjoint of the WHFast
+// democratic-heliocentric step (tides, rotational flattening, Kidder 1995 GR,
+// spin), one segment of recorded steps at a time.
+//
+// Mapping: the forward step's (one lane per body, one lane group per system,
+// several systems per wavefront, one wavefront per workgroup).  Each lane
+// holds the cotangents of its body's position, velocity and spin angular
+// momentum and accumulates the gradients of its body's parameters.  The step
+// being reversed (src/integrator/whfast.rs:235-305) is
+//   K(dt/2) . [T^-1 . Kepler(dt/2) . jump(dt/2)] . interaction(dt) . [jump(dt/2) . Kepler(dt/2) . T] . K(dt/2)
+// with T the democratic-heliocentric transform (whfast.rs:973-1023,
+// 1090-1155) and K the implicit-midpoint kick (whfast.rs:322-466).
+//
+// Discretisation choices of the adjoint:
+//   * the implicit midpoint y1 = y0 + h F(x, (y0 + y1) / 2) is differentiated
+//     as the implicit equation it solves, linearised at (y0 + y1) / 2: the
+//     adjoint equation mu = lambda + (h/2) F_y^T mu is solved by fixed-point
+//     passes to convergence (ADJ_FIXED_POINT_TOL);
+//   * the Kepler drift is differentiated through the universal Kepler
+//     equation at the recorded universal anomaly (implicit function theorem);
+//   * the compensated-summation residues (whfast.rs:117-119) carry no
+//     derivative (they are rounding errors);
+//   * the spins entering the scalar products are L / I at the linearisation
+//     point (the forward uses the spin of the previous midpoint iteration,
+//     which differs from it by the convergence tolerance).
+#pragma once
+
+#include "whfast_step.hpp"
+
+#define POS_HD __device__ inline
+#include "adjoint_math.hpp"
+
+namespace posdev {
+
+constexpr int ADJ_ROWS = 28;
+// The adjoint midpoint equation mu = lambda + (h/2) F_y^T mu is iterated
+// until mu changes by less than this (relative, per lane and block) or for
+// at most ADJ_MAX_PASSES passes.  The contraction factor is h |F_y| / 2:
+// below 1e-7 for the forces, up to ~1e-4 for the tidal spin-down of close
+// planets, so three to five passes are typical.
+constexpr double ADJ_FIXED_POINT_TOL = 1e-14;
+constexpr int ADJ_MAX_PASSES = 12;
+
+struct AdjLds {
+    double m[WAVE], I[WAVE], R[WAVE], sg[WAVE], k2[WAVE], kf[WAVE];
+    double xb[ADJ_ROWS][WAVE];
+};
+
+struct AdjArgs {
+    DevEnsemble E;                  // constants and launch table (state arrays unused)
+    TapeRef T;                      // the segment's tape
+    long long seg_start;            // index of the segment's first step in the recorded run
+    int seg_len;                    // steps in the segment
+    const long long *steps_done;    // [n_systems] steps each system executed in the recorded run
+    double *lx, *lv, *lL;           // [3][n_bodies] cotangents: in = after the segment, out = before it
+    double *gs, *gk2, *gkf;         // [n_bodies] accumulated parameter gradients
+};
+
+struct AdjLane {
+    int lane, gbase, k, n, host, sys, body;
+    bool valid, is_host, tides, flat, gr, spin;
+    double dt, m, I, R, sg, k2, kf, mh, Ih, M;
+    long long done;
+    double lx[3], lv[3], lL[3];
+    double gs, gk2, gkf;
+    __device__ __forceinline__ bool orbiting() const { return valid && !is_host; }
+    __device__ __forceinline__ int hl() const { return gbase + host; }
+};
+
+using posadj::V3;
+
+__device__ __forceinline__ V3 ld3(const double a[3]) { return V3{a[0], a[1], a[2]}; }
+__device__ __forceinline__ void st3(double a[3], V3 v) { a[0] = v.x; a[1] = v.y; a[2] = v.z; }
+
+// Sums over the bodies of the group (order is irrelevant for the adjoint).
+__device__ __forceinline__ V3 gsum3(const AdjLds &L, const AdjLane &A, int row, bool skip_host) {
+    V3 s{0., 0., 0.};
+    for (int j = 0; j < A.n; ++j) {
+        if (skip_host && j == A.host) continue;
+        const int l = A.gbase + j;
+        s.x += L.xb[row][l]; s.y += L.xb[row + 1][l]; s.z += L.xb[row + 2][l];
+    }
+    return s;
+}
+__device__ __forceinline__ V3 gsum3m(const AdjLds &L, const AdjLane &A, int row, bool skip_host) {
+    V3 s{0., 0., 0.};
+    for (int j = 0; j < A.n; ++j) {
+        if (skip_host && j == A.host) continue;
+        const int l = A.gbase + j;
+        const double mj = L.m[l];
+        s.x += mj * L.xb[row][l]; s.y += mj * L.xb[row + 1][l]; s.z += mj * L.xb[row + 2][l];
+    }
+    return s;
+}
+__device__ __forceinline__ void put3(AdjLds &L, int row, int lane, V3 v) {
+    L.xb[row][lane] = v.x; L.xb[row + 1][lane] = v.y; L.xb[row + 2][lane] = v.z;
+}
+__device__ __forceinline__ V3 get3(const AdjLds &L, int row, int lane) {
+    return V3{L.xb[row][lane], L.xb[row + 1][lane], L.xb[row + 2][lane]};
+}
+
+// One pass of the effects' vector-Jacobian product at the linearisation
+// point published in rows 0-8 (x, v, W = L / I of every body): given the
+// cotangents (mu_a, mu_L) of every body's additional acceleration and dL/dt,
+// return this body's cotangents of x, v, L and of its parameters.
+__device__ inline void effects_vjp_pass(AdjLds &L, const AdjLane &A, const posadj::PairConst &C,
+                                        const posadj::PairPar &P, const posadj::PairIn &in, V3 mu_a, V3 mu_L,
+                                        V3 &gx, V3 &gv, V3 &gL, double pg[3]) {
+    const int lane = A.lane;
+    if (A.valid) { put3(L, 9, lane, mu_a); put3(L, 12, lane, mu_L); }
+    wsync();
+    gx = V3{0., 0., 0.}; gv = gx; gL = gx;
+    pg[0] = pg[1] = pg[2] = 0.;
+    if (A.orbiting()) {
+        const int hl = A.hl();
+        const V3 mah = get3(L, 9, hl), mLh = get3(L, 12, hl);
+        posadj::PairOut ob;
+        ob.F = (1. / A.m) * mu_a - (1. / A.mh) * mah;
+        ob.g = mu_a - (A.m / A.mh) * mah;
+        ob.Np = mu_L;
+        ob.Nh = mLh;
+        posadj::PairIn Ib{V3{0., 0., 0.}, V3{0., 0., 0.}, V3{0., 0., 0.}, V3{0., 0., 0.}};
+        posadj::PairParBar Pb{0., 0., 0., 0., 0., 0.};
+        posadj::pair_vjp(C, P, in, ob, Ib, Pb);
+        gx = Ib.r;
+        gv = Ib.u;
+        gL = (1. / A.I) * Ib.Wp;
+        pg[0] = Pb.sp; pg[1] = Pb.k2p; pg[2] = Pb.kfp;
+        put3(L, 15, lane, -Ib.r);
+        put3(L, 18, lane, -Ib.u);
+        put3(L, 21, lane, (1. / A.Ih) * Ib.Wh);
+        put3(L, 24, lane, V3{Pb.sh, Pb.k2h, Pb.kfh});
+    }
+    wsync();
+    if (A.valid && A.is_host) {
+        gx = gsum3(L, A, 15, true);
+        gv = gsum3(L, A, 18, true);
+        gL = gsum3(L, A, 21, true);
+        const V3 p = gsum3(L, A, 24, true);
+        pg[0] = p.x; pg[1] = p.y; pg[2] = p.z;
+    }
+    wsync();
+}
+
+// True when every lane of the group passes `flag` (row 27 is the scratch row).
+__device__ inline bool group_all(AdjLds &L, const AdjLane &A, bool flag) {
+    L.xb[27][A.lane] = flag ? 1. : 0.;
+    wsync();
+    bool all = true;
+    for (int j = 0; j < A.n; ++j) all = all && L.xb[27][A.gbase + j] != 0.;
+    wsync();
+    return all;
+}
+
+__device__ __forceinline__ double maxabs3(V3 a) { return fmax(fabs(a.x), fmax(fabs(a.y), fabs(a.z))); }
+
+// |a - b| small against |a| (both zero counts as equal)
+__device__ __forceinline__ bool settled(V3 a, V3 b) {
+    return maxabs3(a - b) <= ADJ_FIXED_POINT_TOL * maxabs3(a);
+}
+
+// Adjoint of one implicit-midpoint kick (x fixed, y = (v, L), half step h):
+// on entry A.lv / A.lL are the cotangents of y1 and A.lx that of x; on exit
+// A.lv / A.lL are those of y0, A.lx has the kick's contribution added and the
+// parameter gradients are accumulated.
+__device__ inline void kick_adjoint(AdjLds &L, AdjLane &A, V3 x, V3 v0, V3 L0, V3 v1, V3 L1, double h) {
+    const int lane = A.lane;
+    const V3 vm = 0.5 * (v0 + v1);
+    const V3 Lm = A.spin ? 0.5 * (L0 + L1) : L0;
+    const V3 W = (1. / A.I) * Lm;
+    if (A.valid) { put3(L, 0, lane, x); put3(L, 3, lane, vm); put3(L, 6, lane, W); }
+    wsync();
+    const int hl = A.hl();
+    const posadj::PairConst C{A.mh, A.m, L.R[hl], A.R, A.Ih, A.I, A.tides, A.flat, A.gr};
+    const posadj::PairPar P{L.sg[hl], A.sg, L.k2[hl], A.k2, L.kf[hl], A.kf};
+    posadj::PairIn in{V3{0., 0., 0.}, V3{0., 0., 0.}, V3{0., 0., 0.}, W};
+    if (A.orbiting()) {
+        in.r = x - get3(L, 0, hl);
+        in.u = vm - get3(L, 3, hl);
+        in.Wh = get3(L, 6, hl);
+    }
+    const V3 lam_v = ld3(A.lv), lam_L = ld3(A.lL);
+    const V3 z{0., 0., 0.};
+    V3 gx, gv, gL;
+    double pg[3];
+    // fixed point of mu = lambda + (h/2) F_y^T mu, starting from mu = lambda
+    V3 mu_v = lam_v, mu_L = A.spin ? lam_L : z;
+    for (int pass = 0;; ++pass) {
+        effects_vjp_pass(L, A, C, P, in, mu_v, mu_L, gx, gv, gL, pg);
+        const V3 nv = lam_v + (0.5 * h) * gv;
+        const V3 nL = A.spin ? lam_L + (0.5 * h) * gL : z;
+        const bool conv = !A.valid || (settled(nv, mu_v) && settled(nL, mu_L));
+        mu_v = nv;
+        mu_L = nL;
+        if (pass + 1 >= ADJ_MAX_PASSES || group_all(L, A, conv)) break;
+    }
+    // y0's cotangent is mu + (h/2) F_y^T mu = 2 mu - lambda
+    st3(A.lv, 2. * mu_v - lam_v);
+    st3(A.lL, A.spin ? 2. * mu_L - lam_L : lam_L + h * gL);
+    st3(A.lx, ld3(A.lx) + h * gx);
+    A.gs += h * pg[0];
+    A.gk2 += h * pg[1];
+    A.gkf += h * pg[2];
+}
+
+__device__ __forceinline__ V3 tape3(const TapeRef &T, int row, int field, int body) {
+    return V3{*T.at(row, field, body), *T.at(row, field + 1, body), *T.at(row, field + 2, body)};
+}
+
+// Reverse of step `row` of the segment: A.l* hold the cotangents of the
+// state after the step on entry and before it on exit.
+__device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, int row) {
+    const TapeRef &T = Ar.T;
+    const int lane = A.lane;
+    const int b = A.body;
+    const V3 z{0., 0., 0.};
+    V3 x0 = z, v0 = z, L0 = z, v1 = z, L1 = z, x2 = z, v2 = z, L2 = z;
+    double X1 = 0., X2 = 0.;
+    if (A.valid) {
+        x0 = tape3(T, row, TP_X, b); v0 = tape3(T, row, TP_V, b); L0 = tape3(T, row, TP_L, b);
+        v1 = tape3(T, row, TP_V1, b); L1 = tape3(T, row, TP_L1, b);
+        X1 = *T.at(row, TP_XK1, b); X2 = *T.at(row, TP_XK2, b);
+        x2 = tape3(T, row + 1, TP_X, b); v2 = tape3(T, row + 1, TP_V, b); L2 = tape3(T, row + 1, TP_L, b);
+    }
+    const double h = 0.5 * A.dt, dt = A.dt;
+    const double mu = posadj::G_K2 * A.mh;
+    const bool orb = A.orbiting();
+    const int hl = A.hl();
+
+    // ---------------- recompute the drift from (x0, v1) ----------------
+    if (A.valid) { put3(L, 0, lane, x0); put3(L, 3, lane, v1); }
+    wsync();
+    const V3 Xc = (1. / A.M) * gsum3m(L, A, 0, false);
+    const V3 V = (1. / A.M) * gsum3m(L, A, 3, false);
+    const V3 Q0 = orb ? x0 - get3(L, 0, hl) : z;
+    const V3 P0 = orb ? v1 - V : z;
+    wsync();
+    V3 Q1 = z, P1 = z;
+    if (orb) posadj::kepler_forward_x(mu, h, X1, Q0, P0, Q1, P1);
+    const V3 Xc1 = Xc + h * V;
+    if (orb) put3(L, 0, lane, P1);
+    wsync();
+    const V3 Q1p = orb ? Q1 + (h / A.mh) * gsum3m(L, A, 0, true) : z;
+    wsync();
+    if (orb) put3(L, 0, lane, Q1p);
+    wsync();
+    const V3 xh = Xc1 - (1. / A.M) * gsum3m(L, A, 0, true);
+    const V3 xg = orb ? Q1p + xh : xh;
+    wsync();
+    if (A.valid) put3(L, 0, lane, xg);
+    wsync();
+    V3 acc = z;
+    if (orb) {
+        for (int j = 0; j < A.n; ++j) {
+            if (j == A.host || j == A.k) continue;
+            const int l = A.gbase + j;
+            const V3 d = xg - get3(L, 0, l);
+            const double d2 = posadj::dot(d, d);
+            const double id = 1. / sqrt(d2);
+            acc += (-posadj::G_K2 * L.m[l] * id * id * id) * d;
+        }
+    }
+    const V3 P1pp = P1 + dt * acc;
+    wsync();
+    if (orb) put3(L, 0, lane, P1pp);
+    wsync();
+    const V3 Q2 = orb ? Q1p + (h / A.mh) * gsum3m(L, A, 0, true) : z;
+    V3 Q3 = z, P3 = z;
+    if (orb) posadj::kepler_forward_x(mu, h, X2, Q2, P1pp, Q3, P3);
+    wsync();
+    if (orb) put3(L, 0, lane, P3);
+    wsync();
+    const V3 v2p = orb ? P3 + V : V - (1. / A.mh) * gsum3m(L, A, 0, true);
+    wsync();
+
+    // ---------------- reverse ----------------
+    // second kick (at x2, between (v2', L1) and (v2, L2))
+    kick_adjoint(L, A, x2, v2p, L1, v2, L2, h);
+    // T^-1 of the second drift
+    if (A.valid) { put3(L, 0, lane, ld3(A.lx)); put3(L, 3, lane, ld3(A.lv)); }
+    wsync();
+    const V3 xh_tot = gsum3(L, A, 0, false);
+    const V3 lvh = get3(L, 3, hl);
+    const V3 Vb0 = lvh + gsum3(L, A, 3, true);
+    const V3 Q3b = orb ? ld3(A.lx) - (A.m / A.M) * xh_tot : z;
+    const V3 P3b = orb ? ld3(A.lv) - (A.m / A.mh) * lvh : z;
+    wsync();
+    // second Kepler drift and the centre-of-mass drift
+    V3 Q2b = z, P1ppb = z;
+    if (orb) posadj::kepler_vjp(mu, h, X2, Q2, P1pp, Q3b, P3b, Q2b, P1ppb);
+    V3 Xc1b = xh_tot;
+    V3 Vb = Vb0 + h * xh_tot;
+    // second jump
+    if (orb) put3(L, 0, lane, Q2b);
+    wsync();
+    if (orb) P1ppb += (h * A.m / A.mh) * gsum3(L, A, 0, true);
+    V3 Q1pb = Q2b;
+    wsync();
+    // interaction and planet-planet gravity
+    V3 P1b = P1ppb;
+    const V3 ab = dt * P1ppb;
+    if (A.valid) { put3(L, 0, lane, orb ? ab : z); put3(L, 3, lane, xg); }
+    wsync();
+    V3 xgb = z;
+    if (orb) {
+        for (int j = 0; j < A.n; ++j) {
+            if (j == A.host || j == A.k) continue;
+            const int l = A.gbase + j;
+            const V3 d = xg - get3(L, 3, l);
+            xgb += posadj::gravity_pair_vjp(L.m[l], d, ab) - posadj::gravity_pair_vjp(A.m, d, get3(L, 0, l));
+        }
+    }
+    wsync();
+    // positions for gravity: x_p = Q1p_p + x_h, x_h = Xc1 - sum m Q1p / M
+    if (A.valid) put3(L, 0, lane, xgb);
+    wsync();
+    const V3 xgh_tot = gsum3(L, A, 0, true);
+    if (orb) Q1pb += xgb - (A.m / A.M) * xgh_tot;
+    Xc1b += xgh_tot;
+    wsync();
+    // first jump
+    if (orb) put3(L, 0, lane, Q1pb);
+    wsync();
+    if (orb) P1b += (h * A.m / A.mh) * gsum3(L, A, 0, true);
+    wsync();
+    // first Kepler drift and the centre-of-mass drift
+    V3 Q0b = z, P0b = z;
+    if (orb) posadj::kepler_vjp(mu, h, X1, Q0, P0, Q1pb, P1b, Q0b, P0b);
+    const V3 Xcb = Xc1b;
+    Vb += h * Xc1b;
+    // T of the first drift
+    if (orb) { put3(L, 0, lane, Q0b); put3(L, 3, lane, P0b); }
+    wsync();
+    const V3 SQ = gsum3(L, A, 0, true), SP = gsum3(L, A, 3, true);
+    const V3 Vtot = Vb - SP;
+    if (A.valid) {
+        st3(A.lx, (orb ? Q0b : -SQ) + (A.m / A.M) * Xcb);
+        st3(A.lv, (orb ? P0b : z) + (A.m / A.M) * Vtot);
+    }
+    wsync();
+    // first kick (at x0, between (v0, L0) and (v1, L1))
+    kick_adjoint(L, A, x0, v0, L0, v1, L1, h);
+}
+
+__device__ inline void adjoint_load(const AdjArgs &Ar, AdjLds &L, AdjLane &A, int wave) {
+    const DevEnsemble &E = Ar.E;
+    const int lane = threadIdx.x;
+    const int G = E.wave_group[wave];
+    const int g = lane / G;
+    const int k = lane - g * G;
+    const bool gvalid = g < E.wave_count[wave];
+    const int s = E.wave_first[wave] + (gvalid ? g : 0);
+    A.lane = lane;
+    A.gbase = g * G;
+    A.k = k;
+    A.sys = s;
+    A.n = gvalid ? (E.body_offset[s + 1] - E.body_offset[s]) : 0;
+    A.valid = gvalid && k < A.n;
+    A.host = gvalid ? E.host[s] : 0;
+    A.is_host = A.valid && k == A.host;
+    A.body = E.body_offset[s] + (A.valid ? k : 0);
+    const uint32_t eff = E.effects[s];
+    A.tides = eff & POS_EFFECT_TIDES;
+    A.flat = eff & POS_EFFECT_ROTATIONAL_FLATTENING;
+    A.gr = eff & POS_EFFECT_GENERAL_RELATIVITY;
+    A.spin = eff & (POS_EFFECT_TIDES | POS_EFFECT_ROTATIONAL_FLATTENING | POS_EFFECT_GENERAL_RELATIVITY | POS_EFFECT_EVOLUTION);
+    A.dt = E.dt[s];
+    A.done = Ar.steps_done[s];
+    const int b = A.body;
+    const int nb = E.n_bodies;
+    A.m = A.valid ? E.mass[b] : 0.;
+    A.I = E.inertia[b];
+    A.R = E.radius[b];
+    A.sg = E.sigma[b];
+    A.k2 = E.k2[b];
+    A.kf = E.k2f[b];
+    L.m[lane] = A.m;
+    L.I[lane] = A.I;
+    L.R[lane] = A.R;
+    L.sg[lane] = A.sg;
+    L.k2[lane] = A.k2;
+    L.kf[lane] = A.kf;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        A.lx[q] = A.valid ? Ar.lx[q * nb + b] : 0.;
+        A.lv[q] = A.valid ? Ar.lv[q * nb + b] : 0.;
+        A.lL[q] = A.valid ? Ar.lL[q * nb + b] : 0.;
+    }
+    A.gs = A.valid ? Ar.gs[b] : 0.;
+    A.gk2 = A.valid ? Ar.gk2[b] : 0.;
+    A.gkf = A.valid ? Ar.gkf[b] : 0.;
+    __syncthreads();
+    A.mh = L.m[A.hl()];
+    A.Ih = L.I[A.hl()];
+    double M = 0.;
+    for (int j = 0; j < A.n; ++j) M += L.m[A.gbase + j];
+    A.M = M;
+}
+
+__device__ inline void adjoint_store(const AdjArgs &Ar, const AdjLane &A) {
+    if (!A.valid) return;
+    const int b = A.body;
+    const int nb = Ar.E.n_bodies;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        Ar.lx[q * nb + b] = A.lx[q];
+        Ar.lv[q * nb + b] = A.lv[q];
+        Ar.lL[q * nb + b] = A.lL[q];
+    }
+    Ar.gs[b] = A.gs;
+    Ar.gk2[b] = A.gk2;
+    Ar.gkf[b] = A.gkf;
+}
+
+}  // namespace posdev

@@ -382,17 +382,22 @@ POS_HD void stiefel_g05(double beta, double X, double g[6]) {
     const double z = beta * X * X;
     double c[6];
     if (fabs(z) < 1.) {
-        // c_k(z) = sum_j (-z)^j / (k + 2j)!
-        for (int k = 0; k < 6; ++k) {
-            double fact = 1.;
-            for (int i = 2; i <= k; ++i) fact *= (double)i;
-            double term = 1. / fact, sum = 0.;
-            for (int j = 0; j < 18; ++j) {
-                sum += term;
-                term *= -z / ((double)(k + 2 * j + 1) * (double)(k + 2 * j + 2));
-            }
-            c[k] = sum;
+        // c_k(z) = sum_j (-z)^j / (k + 2j)! for k = 4, 5 (12 terms: the
+        // remainder is below 1/28! for |z| < 1), then c_k = 1/k! - z c_{k+2}
+        // downwards, which only multiplies by |z| < 1.
+        double t4 = 1. / 24., t5 = 1. / 120., s4 = 0., s5 = 0.;
+        for (int j = 0; j < 12; ++j) {
+            s4 += t4;
+            s5 += t5;
+            t4 *= -z / ((double)(5 + 2 * j) * (double)(6 + 2 * j));
+            t5 *= -z / ((double)(6 + 2 * j) * (double)(7 + 2 * j));
         }
+        c[4] = s4;
+        c[5] = s5;
+        c[3] = 1. / 6. - z * c[5];
+        c[2] = 0.5 - z * c[4];
+        c[1] = 1. - z * c[3];
+        c[0] = 1. - z * c[2];
     } else if (z > 0.) {
         const double s = sqrt(z);
         c[0] = cos(s);

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "whfast_step.hpp"
+#include "adjoint_kernels.hpp"
 
 #ifndef POS_WAVES_PER_EU
 #define POS_WAVES_PER_EU 2   // occupancy target of the step kernel (waves per SIMD)
@@ -146,11 +147,44 @@ void whfast_steps_kernel(DevEnsemble E, int n_steps) {
     for (int wave = blockIdx.x; wave < E.n_waves; wave += gridDim.x) {
         Lane B;
         load_lane(E, lds, B, wave);
+        const TapeRef none{nullptr, 0};
         for (int step = 0; step < n_steps; ++step) {
             if (B.status & FROZEN) break;   // uniform within the group
-            iterate(E, lds, B);
+            iterate<false>(E, lds, B, none, 0);
         }
         store_lane(E, lds, B);
+        __syncthreads();
+    }
+}
+
+// The same K steps, also writing the step tape of the adjoint (one row per
+// step plus the final state).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POS_WAVES_PER_EU)))
+void whfast_record_kernel(DevEnsemble E, int n_steps, TapeRef T) {
+    __shared__ WaveLds lds;
+    for (int wave = blockIdx.x; wave < E.n_waves; wave += gridDim.x) {
+        Lane B;
+        load_lane(E, lds, B, wave);
+        for (int step = 0; step < n_steps; ++step) {
+            if (B.status & FROZEN) break;   // uniform within the group
+            iterate<true>(E, lds, B, T, step);
+        }
+        store_lane(E, lds, B);
+        __syncthreads();
+    }
+}
+
+// Reverse sweep over one recorded segment, last step first.
+__global__ __launch_bounds__(64) void adjoint_segment_kernel(AdjArgs Ar) {
+    __shared__ AdjLds L;
+    for (int wave = blockIdx.x; wave < Ar.E.n_waves; wave += gridDim.x) {
+        AdjLane A;
+        adjoint_load(Ar, L, A, wave);
+        for (int j = Ar.seg_len - 1; j >= 0; --j) {
+            if (Ar.seg_start + j >= A.done) continue;   // uniform within the group
+            adjoint_step(Ar, L, A, j);
+        }
+        adjoint_store(Ar, A);
         __syncthreads();
     }
 }
@@ -256,6 +290,21 @@ struct pos_ens {
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_launches = 0;
+    // per system / per body kinds in device order (adjoint support checks)
+    std::vector<uint32_t> eff_dev;
+    std::vector<int> coord_dev, evo_dev;
+    // discrete adjoint: checkpoints of the recorded run and the segment tape
+    struct Adjoint {
+        bool recorded = false;
+        int64_t n_steps = 0;
+        int seg = 0, n_seg = 0;
+        size_t state_bytes = 0;
+        char *ckpt = nullptr;              // (n_seg + 1) states: segment starts, then the final state
+        double *tape = nullptr;            // (seg + 1) * TAPE_FIELDS * n_bodies
+        long long *iter0 = nullptr, *steps_done = nullptr;
+        double *lx = nullptr, *lv = nullptr, *lL = nullptr, *gs = nullptr, *gk2 = nullptr, *gkf = nullptr;
+        std::vector<void *> allocs;
+    } adj;
 };
 
 namespace {
@@ -421,6 +470,30 @@ double host_roche(double ma, double mb, double ra, double rb) {
     // universe.rs:186-193 (setup only, evaluated with the host libm)
     if (ma > mb) return (rb / 0.462) * std::pow(ma / mb, 1. / 3.);
     return (ra / 0.462) * std::pow(mb / ma, 1. / 3.);
+}
+
+// The mutable device state of an ensemble (what a checkpoint must hold).
+std::vector<std::pair<void *, size_t>> adjoint_state_fields(pos_ens *h) {
+    const size_t nb = h->n_bodies, ns = h->n_systems;
+    DevEnsemble &E = h->E;
+    const size_t v3 = 3 * nb * sizeof(double), s1 = nb * sizeof(double);
+    return {{E.x, v3}, {E.v, v3}, {E.L, v3}, {E.spin, v3}, {E.err_v, v3}, {E.err_L, v3}, {E.acc, v3},
+            {E.radius, s1}, {E.rg2, s1}, {E.inertia, s1}, {E.left_index, nb * sizeof(int)},
+            {E.t, ns * sizeof(double)}, {E.last_hist, ns * sizeof(double)}, {E.iteration, ns * sizeof(long long)},
+            {E.status, ns * sizeof(uint32_t)}, {E.timestep_warning, ns * sizeof(int)}};
+}
+
+// Copy the mutable state into (save) or out of (restore) a checkpoint slot.
+int adjoint_copy_state(pos_ens *h, char *slot, bool save) {
+    size_t off = 0;
+    for (const auto &f : adjoint_state_fields(h)) {
+        if (save)
+            HIP_TRY(hipMemcpyAsync(slot + off, f.first, f.second, hipMemcpyDeviceToDevice, h->stream));
+        else
+            HIP_TRY(hipMemcpyAsync(f.first, slot + off, f.second, hipMemcpyDeviceToDevice, h->stream));
+        off += (f.second + 255) / 256 * 256;
+    }
+    return POS_OK;
 }
 
 constexpr int GRAPH_LAUNCHES = 4;   // kernel launches per captured graph
@@ -617,6 +690,9 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     }
     if ((rc = build_launch(h, o))) { pos_destroy(h); return rc; }
     if (hipStreamSynchronize(h->stream) != hipSuccess) { pos_destroy(h); return fail(POS_E_HIP, "synchronize failed"); }
+    h->eff_dev = eff;
+    h->coord_dev = coord;
+    h->evo_dev = evo_type;
     *out = h;
     return POS_OK;
 }
@@ -949,6 +1025,138 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
     return POS_OK;
 }
 
+int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
+    if (!h) return fail(POS_E_INVALID, "handle is NULL");
+    if (n_steps <= 0) return fail(POS_E_INVALID, "n_steps must be positive");
+    if (segment_steps < 0) return fail(POS_E_INVALID, "segment_steps must be >= 0");
+    for (int ds = 0; ds < h->n_systems; ++ds) {
+        const std::string where = "system " + std::to_string(h->sys_perm[ds]) + ": ";
+        if (h->coord_dev[ds] != POS_COORD_DEMOCRATIC_HELIOCENTRIC)
+            return fail(POS_E_UNSUPPORTED, where + "the adjoint supports democratic-heliocentric coordinates only");
+        if (h->eff_dev[ds] & (POS_EFFECT_WIND | POS_EFFECT_DISK | POS_EFFECT_EVOLUTION))
+            return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support wind, disk or evolution");
+        for (int b = h->body_offset_dev[ds]; b < h->body_offset_dev[ds + 1]; ++b)
+            if (h->evo_dev[b] != POS_EVO_NON_EVOLVING)
+                return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support evolving bodies");
+    }
+    HIP_TRY(hipSetDevice(h->device));
+    pos_ens::Adjoint &A = h->adj;
+    for (void *p : A.allocs) (void)hipFree(p);
+    A = pos_ens::Adjoint();
+    const int64_t seg64 = std::min<int64_t>(segment_steps > 0 ? segment_steps : 1000, n_steps);
+    if (seg64 > (1 << 24)) return fail(POS_E_LIMIT, "segment_steps too large");
+    const int seg = (int)seg64;
+    const int64_t n_seg = (n_steps + seg - 1) / seg;
+    const size_t nb = h->n_bodies, ns = h->n_systems;
+    const std::vector<std::pair<void *, size_t>> fields = adjoint_state_fields(h);
+    size_t state_bytes = 0;
+    for (const auto &f : fields) state_bytes += (f.second + 255) / 256 * 256;
+    const size_t ckpt_bytes = state_bytes * (size_t)(n_seg + 1);
+    const size_t tape_bytes = (size_t)(seg + 1) * TAPE_FIELDS * nb * sizeof(double);
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    if (ckpt_bytes + tape_bytes + 64 * nb * sizeof(double) > free_b * 9 / 10)
+        return fail(POS_E_LIMIT, "checkpoints (" + std::to_string(ckpt_bytes >> 20) + " MiB) and segment tape (" +
+                                 std::to_string(tape_bytes >> 20) + " MiB) exceed free device memory; use fewer steps "
+                                 "per segment or fewer systems");
+    auto aalloc = [&](void **p, size_t bytes) -> int {
+        if (hipMalloc(p, std::max<size_t>(bytes, 256)) != hipSuccess) return fail(POS_E_HIP, "hipMalloc failed (adjoint)");
+        A.allocs.push_back(*p);
+        return POS_OK;
+    };
+    int rc;
+    if ((rc = aalloc((void **)&A.ckpt, ckpt_bytes)) || (rc = aalloc((void **)&A.tape, tape_bytes)) ||
+        (rc = aalloc((void **)&A.iter0, ns * sizeof(long long))) ||
+        (rc = aalloc((void **)&A.steps_done, ns * sizeof(long long))) ||
+        (rc = aalloc((void **)&A.lx, 3 * nb * sizeof(double))) || (rc = aalloc((void **)&A.lv, 3 * nb * sizeof(double))) ||
+        (rc = aalloc((void **)&A.lL, 3 * nb * sizeof(double))) || (rc = aalloc((void **)&A.gs, nb * sizeof(double))) ||
+        (rc = aalloc((void **)&A.gk2, nb * sizeof(double))) || (rc = aalloc((void **)&A.gkf, nb * sizeof(double))))
+        return rc;
+    A.seg = seg;
+    A.n_seg = (int)n_seg;
+    A.n_steps = n_steps;
+    A.state_bytes = state_bytes;
+    HIP_TRY(hipMemcpyAsync(A.iter0, h->E.iteration, ns * sizeof(long long), hipMemcpyDeviceToDevice, h->stream));
+    for (int64_t s = 0; s < n_seg; ++s) {
+        if ((rc = adjoint_copy_state(h, A.ckpt + (size_t)s * state_bytes, true))) return rc;
+        const int64_t len = std::min<int64_t>(seg, n_steps - s * seg);
+        if ((rc = pos_step(h, len))) return rc;
+    }
+    if ((rc = adjoint_copy_state(h, A.ckpt + (size_t)n_seg * state_bytes, true))) return rc;
+    std::vector<long long> i0(ns), i1(ns);
+    if ((rc = download(h, i0, A.iter0, ns)) || (rc = download(h, i1, h->E.iteration, ns))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<long long> done(ns);
+    for (size_t s = 0; s < ns; ++s) done[s] = i1[s] - i0[s];
+    if ((rc = upload(h, A.steps_done, done))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    A.recorded = true;
+    return POS_OK;
+}
+
+int32_t pos_adjoint(pos_ens *h, const pos_adjoint_seed *seed, pos_adjoint_grad *grad) {
+    if (!h) return fail(POS_E_INVALID, "handle is NULL");
+    if (!seed || !grad) return fail(POS_E_INVALID, "seed and grad must not be NULL");
+    pos_ens::Adjoint &A = h->adj;
+    if (!A.recorded) return fail(POS_E_INVALID, "no recorded run: call pos_adjoint_record first");
+    HIP_TRY(hipSetDevice(h->device));
+    const int nb = h->n_bodies;
+    int rc;
+    // seeds, input -> device order
+    {
+        std::vector<double> lx(3 * (size_t)nb, 0.), lv(3 * (size_t)nb, 0.), lL(3 * (size_t)nb, 0.), zero(nb, 0.);
+        for (int db = 0; db < nb; ++db) {
+            const int b = h->body_perm[db];
+            for (int q = 0; q < 3; ++q) {
+                if (seed->position) lx[(size_t)q * nb + db] = seed->position[3 * (size_t)b + q];
+                if (seed->velocity) lv[(size_t)q * nb + db] = seed->velocity[3 * (size_t)b + q];
+                if (seed->angular_momentum) lL[(size_t)q * nb + db] = seed->angular_momentum[3 * (size_t)b + q];
+            }
+        }
+        if ((rc = upload(h, A.lx, lx)) || (rc = upload(h, A.lv, lv)) || (rc = upload(h, A.lL, lL)) ||
+            (rc = upload(h, A.gs, zero)) || (rc = upload(h, A.gk2, zero)) || (rc = upload(h, A.gkf, zero)))
+            return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));   // the host vectors go out of scope
+    }
+    AdjArgs Ar{};
+    Ar.E = h->E;
+    Ar.T = TapeRef{A.tape, nb};
+    Ar.steps_done = A.steps_done;
+    Ar.lx = A.lx; Ar.lv = A.lv; Ar.lL = A.lL;
+    Ar.gs = A.gs; Ar.gk2 = A.gk2; Ar.gkf = A.gkf;
+    for (int s = A.n_seg - 1; s >= 0; --s) {
+        const int64_t start = (int64_t)s * A.seg;
+        const int len = (int)std::min<int64_t>(A.seg, A.n_steps - start);
+        if ((rc = adjoint_copy_state(h, A.ckpt + (size_t)s * A.state_bytes, false))) return rc;
+        hipLaunchKernelGGL(whfast_record_kernel, dim3(h->grid), dim3(64), 0, h->stream, h->E, len, Ar.T);
+        HIP_TRY(hipGetLastError());
+        Ar.seg_start = start;
+        Ar.seg_len = len;
+        hipLaunchKernelGGL(adjoint_segment_kernel, dim3(h->grid), dim3(64), 0, h->stream, Ar);
+        HIP_TRY(hipGetLastError());
+    }
+    if ((rc = adjoint_copy_state(h, A.ckpt + (size_t)A.n_seg * A.state_bytes, false))) return rc;
+    // results, device -> input order
+    std::vector<double> lx, lv, lL, gs, gk2, gkf;
+    if ((rc = download(h, lx, A.lx, 3 * (size_t)nb)) || (rc = download(h, lv, A.lv, 3 * (size_t)nb)) ||
+        (rc = download(h, lL, A.lL, 3 * (size_t)nb)) || (rc = download(h, gs, A.gs, nb)) ||
+        (rc = download(h, gk2, A.gk2, nb)) || (rc = download(h, gkf, A.gkf, nb)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int db = 0; db < nb; ++db) {
+        const int b = h->body_perm[db];
+        for (int q = 0; q < 3; ++q) {
+            if (grad->position) grad->position[3 * (size_t)b + q] = lx[(size_t)q * nb + db];
+            if (grad->velocity) grad->velocity[3 * (size_t)b + q] = lv[(size_t)q * nb + db];
+            if (grad->angular_momentum) grad->angular_momentum[3 * (size_t)b + q] = lL[(size_t)q * nb + db];
+        }
+        if (grad->dissipation_factor) grad->dissipation_factor[b] = gs[db];
+        if (grad->love_number) grad->love_number[b] = gk2[db];
+        if (grad->fluid_love_number) grad->fluid_love_number[b] = gkf[db];
+    }
+    return POS_OK;
+}
+
 void pos_destroy(pos_ens *h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
@@ -956,6 +1164,7 @@ void pos_destroy(pos_ens *h) {
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     for (void *p : h->allocs) (void)hipFree(p);
+    for (void *p : h->adj.allocs) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }

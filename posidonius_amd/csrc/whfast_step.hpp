@@ -139,6 +139,7 @@ struct Lane {
     // alternative coordinates of this body; com* hold the host slot (centre of mass)
     double ax[3], av[3];
     double comx[3], comv[3], com_m;
+    double kx;   // universal anomaly of this body's last Kepler drift
 
     __device__ __forceinline__ bool valid() const { return flags & F_VALID; }
     __device__ __forceinline__ bool is_host() const { return flags & F_HOST; }
@@ -751,6 +752,7 @@ __device__ inline void kepler_individual_step(Lane &B, double mass_g, double dt)
         B.ax[q] = B.ax[q] + (f * x0[q] + g * v0[q]);
         B.av[q] = B.av[q] + (fd * x0[q] + gd * v0[q]);
     }
+    B.kx = x;
 }
 
 // Jacobi prefix sums up to body k (whfast.rs:574-576, 642-651): the
@@ -1114,9 +1116,35 @@ __device__ inline void merge_status(WaveLds &lds, Lane &B) {
 }
 
 // ===========================================================================
-// WHFast::iterate (whfast.rs:235-305)
+// Step tape: what the reverse sweep of the adjoint needs from each step,
+// written by the recording variant of the step.  Row r of a segment holds,
+// per body (component-major, body index fastest), the state at the start of
+// step r (x, v, L), the velocity and angular momentum after the first kick
+// (v1, L1) and the universal anomalies of the two Kepler drifts.  Row
+// n_steps holds the state after the last step in its x, v, L fields.
 // ===========================================================================
-__device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B) {
+constexpr int TAPE_FIELDS = 17;
+enum TapeField { TP_X = 0, TP_V = 3, TP_L = 6, TP_V1 = 9, TP_L1 = 12, TP_XK1 = 15, TP_XK2 = 16 };
+
+struct TapeRef {
+    double *base;      // [rows][TAPE_FIELDS][n_bodies]
+    int n_bodies;
+    __device__ __forceinline__ double *at(int row, int field, int body) const {
+        return base + ((size_t)row * TAPE_FIELDS + field) * (size_t)n_bodies + body;
+    }
+};
+
+__device__ __forceinline__ void tape_put3(const TapeRef &T, int row, int field, int body, const double v[3]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) *T.at(row, field + q, body) = v[q];
+}
+
+// ===========================================================================
+// WHFast::iterate (whfast.rs:235-305).  With REC, the step also writes its
+// tape row `row` (and the end state into row + 1).
+// ===========================================================================
+template <bool REC>
+__device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, const TapeRef &T, int row) {
     {
         const bool first = B.last_hist < 0.;
         const double period = E.hist_period[B.sys];
@@ -1128,17 +1156,33 @@ __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B) {
         }
     }
     const double half_dt = 0.5 * B.dt;
+    if (REC && B.valid()) {
+        tape_put3(T, row, TP_X, B.body, B.x);
+        tape_put3(T, row, TP_V, B.body, B.v);
+        tape_put3(T, row, TP_L, B.body, B.L);
+    }
     integrate_velocity_dependent_forces(E, lds, B, true);
+    if (REC && B.valid()) {
+        tape_put3(T, row, TP_V1, B.body, B.v);
+        tape_put3(T, row, TP_L1, B.body, B.L);
+    }
     inertial_to_alternative(lds, B);
     kepler_steps(lds, B, half_dt);
+    if (REC && B.valid()) *T.at(row, TP_XK1, B.body) = B.orbiting() ? B.kx : 0.;
     jump_step(lds, B, half_dt);
     alternative_to_inertial(lds, B);
     gravity(E, lds, B);
     interaction_step(lds, B, B.dt);
     jump_step(lds, B, half_dt);
     kepler_steps(lds, B, half_dt);
+    if (REC && B.valid()) *T.at(row, TP_XK2, B.body) = B.orbiting() ? B.kx : 0.;
     alternative_to_inertial(lds, B);
     integrate_velocity_dependent_forces(E, lds, B, false);
+    if (REC && B.valid()) {
+        tape_put3(T, row + 1, TP_X, B.body, B.x);
+        tape_put3(T, row + 1, TP_V, B.body, B.v);
+        tape_put3(T, row + 1, TP_L, B.body, B.L);
+    }
     B.t = B.t + B.dt;
     B.iteration += 1;
     merge_status(lds, B);

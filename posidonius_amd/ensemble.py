@@ -111,3 +111,34 @@ class Ensemble:
         dl = np.zeros((self.n_bodies, 3))
         abi.check(self._lib.pos_additional_effects(self._h, abi.ptr(acc, C.c_double), abi.ptr(dl, C.c_double)))
         return acc, dl
+
+    # -- discrete adjoint -----------------------------------------------------
+    def adjoint_record(self, n_steps: int, segment_steps: int = 0):
+        """Advance n_steps like ``step`` and keep a checkpoint every
+        ``segment_steps`` steps (0 = 1000) for ``adjoint``."""
+        abi.check(self._lib.pos_adjoint_record(self._h, C.c_int64(int(n_steps)), int(segment_steps)))
+
+    def adjoint(self, position=None, velocity=None, angular_momentum=None) -> Dict[str, np.ndarray]:
+        """Vector-Jacobian product of the recorded run.  The arguments are the
+        cotangents of the final position, velocity and spin angular momentum
+        ([n_bodies, 3], None = 0).  Returns the cotangents of the initial
+        state and the gradients with respect to every body's dissipation
+        factor sigma (scale * factor), tidal Love number and fluid Love number."""
+        nb = self.n_bodies
+        keep = []
+
+        def src(a):
+            if a is None:
+                return C.cast(None, C.POINTER(C.c_double))
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(nb, 3)
+            keep.append(a)
+            return abi.ptr(a, C.c_double)
+
+        seed = abi.AdjointSeed(src(position), src(velocity), src(angular_momentum))
+        out = {"position": np.zeros((nb, 3)), "velocity": np.zeros((nb, 3)), "angular_momentum": np.zeros((nb, 3)),
+               "dissipation_factor": np.zeros(nb), "love_number": np.zeros(nb), "fluid_love_number": np.zeros(nb)}
+        grad = abi.AdjointGrad(*(abi.ptr(out[k], C.c_double) for k in
+                                 ("position", "velocity", "angular_momentum", "dissipation_factor", "love_number",
+                                  "fluid_love_number")))
+        abi.check(self._lib.pos_adjoint(self._h, C.byref(seed), C.byref(grad)))
+        return out

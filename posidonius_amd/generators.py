@@ -49,6 +49,10 @@ ROLES_ORBITING = abi.ROLE_TIDES_ORBITING | abi.ROLE_FLAT_ORBITING | abi.ROLE_GR_
 FULL_EFFECTS = abi.EFFECT_TIDES | abi.EFFECT_ROTATIONAL_FLATTENING | abi.EFFECT_GENERAL_RELATIVITY
 
 
+# osculating elements kept per body by planetary_system (host rows zero)
+ELEMENT_KEYS = ("a", "e", "inc", "varpi", "node", "mean_anomaly")
+
+
 def planet_dissipation(radius, k2dt):
     return 2.0 * K2 * k2dt / (3.0 * radius ** 5)
 
@@ -184,6 +188,7 @@ def planetary_system(rng, n_systems, n_planets, star_mass, planet_mass, a1, peri
     sigma = np.zeros((ns, nmax))
     sscale = np.ones((ns, nmax))
     k2 = np.zeros((ns, nmax))
+    elements = {k: np.zeros((ns, nmax)) for k in ELEMENT_KEYS}
     # star
     mass[:, 0] = star_mass
     radius[:, 0] = _star_radius(star_mass) if star_radius is None else star_radius
@@ -217,11 +222,16 @@ def planetary_system(rng, n_systems, n_planets, star_mass, planet_mass, a1, peri
         sigma[:, j] = planet_dissipation(rp, K2DT_TERRESTRIAL)
         sscale[:, j] = sigma_scale[:, j - 1]
         k2[:, j] = k2_planet[:, j - 1]
+        for key, val in zip(ELEMENT_KEYS, (a, e, inc, omega + node, node, M)):
+            elements[key][:, j] = np.where(live, val, 0.0)
     if dt is None:
         p_inner = TWO_PI * np.sqrt(a1 ** 3 / (G * (star_mass + planet_mass[:, 0])))
         dt = dt_fraction * p_inner
-    return build_spec(counts, mass, radius, rg2, pos, vel, spin, roles, sigma, sscale, k2, k2.copy(),
+    spec = build_spec(counts, mass, radius, rg2, pos, vel, spin, roles, sigma, sscale, k2, k2.copy(),
                       dt, abi.COORD_DEMOCRATIC_HELIOCENTRIC, names=names)
+    live = np.arange(nmax)[None, :] < counts[:, None]
+    spec.elements = {k: np.ascontiguousarray(v[live]) for k, v in elements.items()}
+    return spec
 
 
 def config0(seed: int = 1000) -> EnsembleSpec:
@@ -286,6 +296,7 @@ def config3(n_systems: int = 16384, seed: int = 1003) -> EnsembleSpec:
     sigma = np.zeros((ns, nmax))
     sscale = np.ones((ns, nmax))
     k2 = np.zeros((ns, nmax))
+    elements = {k: np.zeros((ns, nmax)) for k in ELEMENT_KEYS}
     # stars: star 1 is the central body of every effect, star 2 orbits it
     abin = np.exp(rng.uniform(np.log(0.05), np.log(0.2), ns))
     ebin = rng.uniform(0.0, 0.2, ns)
