@@ -90,6 +90,43 @@ def measured_traffic(workload, steps_per_launch):
     return None
 
 
+def adjoint_cost(n_systems, n_steps, segment_steps):
+    """Cost of the discrete adjoint against the forward pass on the adjoint
+    workload (star + 3 planets, DH coordinates, tides + flattening + GR):
+    forward = pos_step over n_steps; recording = the same steps with a state
+    checkpoint every segment_steps; backward = pos_adjoint (recomputation of
+    every segment with its step tape, then the reverse sweep)."""
+    import numpy as np
+    from posidonius_amd import generators as gen
+    from posidonius_amd.ensemble import Ensemble
+    spec = gen.config4(n_systems)
+    with Ensemble(spec, mapping="packed", use_graph=True) as ens:
+        ens.step(1)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.step(n_steps)
+        ens.synchronize()
+        fwd = time.perf_counter() - t0
+    with Ensemble(spec, mapping="packed", use_graph=True) as ens:
+        ens.step(1)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.adjoint_record(n_steps, segment_steps)
+        rec = time.perf_counter() - t0
+        rng = np.random.default_rng(0)
+        seed = rng.standard_normal((spec.n_bodies, 3))
+        t0 = time.perf_counter()
+        g = ens.adjoint(position=seed, velocity=seed, angular_momentum=None)
+        bwd = time.perf_counter() - t0
+    finite = bool(np.all(np.isfinite(g["position"])) and np.all(np.isfinite(g["dissipation_factor"])))
+    return {"workload": f"{n_systems} star + 3 planet systems (tides, flattening, Kidder GR, spin), "
+                        f"{n_steps} steps, checkpoint every {segment_steps} steps",
+            "forward_s": fwd, "record_s": rec, "backward_s": bwd,
+            "record_over_forward": rec / fwd, "backward_over_forward": bwd / fwd,
+            "forward_system_steps_per_s": n_systems * n_steps / fwd, "gradients_finite": finite,
+            "target_backward_over_forward": 3.0}
+
+
 def cpu_baseline(spec, threads, target_s=12.0):
     """The oracle (C++ restatement of the reference step) on the host cores:
     one system per thread at a time, `threads` threads, a bounded sample."""
@@ -123,6 +160,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-systems", type=int, default=1024)
+    ap.add_argument("--adjoint-steps", type=int, default=100000)
+    ap.add_argument("--adjoint-segment", type=int, default=1000)
     ap.add_argument("--group-size", type=int, default=0)
     ap.add_argument("--steps-per-launch", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -203,6 +244,14 @@ def main():
                                                       "state in HBM between steps"}
         log(f"plain: {prate:.4g} system-steps/s ({pwall / args.plain_steps * 1e3:.2f} ms/step)")
 
+    # ---- differentiable form: adjoint / forward cost ---------------------------
+    adj = None
+    if not args.no_adjoint:
+        log(f"adjoint cost: {args.adjoint_systems} systems x {args.adjoint_steps} steps")
+        adj = adjoint_cost(args.adjoint_systems, args.adjoint_steps, args.adjoint_segment)
+        log(f"adjoint: forward {adj['forward_s']:.2f} s, record {adj['record_s']:.2f} s, "
+            f"backward {adj['backward_s']:.2f} s ({adj['backward_over_forward']:.2f}x forward)")
+
     # ---- CPU baseline ---------------------------------------------------------
     cpu = None
     if not args.no_cpu_baseline:
@@ -258,6 +307,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_gpu": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
+        "adjoint": adj,
         "frozen_systems": frozen,
         "pcie_inclusive": {"value": n_sys * args.steps / (wall + readback_s), "unit": "system-steps/s",
                            "readback_ms": readback_s * 1e3,

@@ -97,6 +97,7 @@ extern "C" {
 #define POS_STATUS_TIMESTEP_WARNING   (1u << 6) /* whfast.rs:702-707 (warning) */
 #define POS_STATUS_KEPLER_BISECTION   (1u << 7) /* whfast.rs:776-808 fallback used */
 #define POS_STATUS_ITERATION_CAP      (1u << 8) /* a loop the reference leaves unbounded hit its cap */
+#define POS_STATUS_HISTORY_OVERFLOW   (1u << 9) /* a historic snapshot found its buffer full (dropped) */
 #define POS_STATUS_FATAL (POS_STATUS_ROCHE | POS_STATUS_COLLISION | POS_STATUS_EJECTION | \
                           POS_STATUS_ZERO_INERTIA | POS_STATUS_ITERATION_CAP)
 
@@ -191,7 +192,8 @@ typedef struct pos_state {
 typedef struct pos_options {
     int32_t device;            /* HIP device ordinal                          */
     int32_t mapping;           /* POS_MAPPING_*                               */
-    int32_t group_size;        /* lanes per system for PACKED (0 = smallest power of two >= N) */
+    int32_t group_size;        /* lanes per system for PACKED (0 = smallest power of two >= N,
+                                  -1 = exactly N, > 0 = that many, at least N) */
     int32_t steps_per_launch;  /* K for PACKED (0 = default 64)               */
     int32_t use_graph;         /* capture chunk sequences in a hipGraph       */
     int32_t reserved[3];
@@ -258,6 +260,31 @@ int32_t pos_tune(pos_ens *ens, int32_t probe_steps, pos_options *chosen);
  * checks are OR-ed into out_status[n_systems] (optional). */
 int32_t pos_gravity_acceleration(pos_ens *ens, double *out_acc, uint32_t *out_status);
 int32_t pos_additional_effects(pos_ens *ens, double *out_acc, double *out_dangular_momentum_dt);
+
+/*
+ * Historic snapshots (src/integrator/output.rs:90-172, written by
+ * WHFast::iterate at whfast.rs:240-256).  After pos_history_enable, every
+ * step whose snapshot condition fires appends one record per body of its
+ * system to a device buffer of `capacity` records per system (a full buffer
+ * drops the record and sets POS_STATUS_HISTORY_OVERFLOW).
+ * pos_history_drain copies the pending records out (input order: system,
+ * then record, then body) and clears the buffer; *n_out receives the count.
+ * The fields are those of the reference's history file; lag_angle and
+ * migration_timescale are 0 (no evolving dissipation, no disk) and
+ * denergy_dt (tides/common.rs:263-281) is evaluated at the snapshot state.
+ */
+typedef struct pos_history_record {
+    int32_t system;            /* input system index                      */
+    int32_t body;              /* index of the body within its system     */
+    double time, time_step;    /* days                                    */
+    double position[3];        /* inertial, AU                            */
+    double spin[3];            /* rad/day                                 */
+    double velocity[3];        /* inertial, AU/day                        */
+    double mass, radius, radius_of_gyration_2;
+    double love_number, scaled_dissipation_factor, lag_angle, denergy_dt, migration_timescale;
+} pos_history_record;
+int32_t pos_history_enable(pos_ens *ens, int32_t capacity);
+int32_t pos_history_drain(pos_ens *ens, pos_history_record *out, int64_t max_records, int64_t *n_out);
 
 /*
  * Discrete adjoint (the differentiable form).  Supported systems: democratic

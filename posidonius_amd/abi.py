@@ -67,6 +67,7 @@ STATUS_MIDPOINT_NOT_CONVERGED = 1 << 5
 STATUS_TIMESTEP_WARNING = 1 << 6
 STATUS_KEPLER_BISECTION = 1 << 7
 STATUS_ITERATION_CAP = 1 << 8
+STATUS_HISTORY_OVERFLOW = 1 << 9
 STATUS_FATAL = STATUS_ROCHE | STATUS_COLLISION | STATUS_EJECTION | STATUS_ZERO_INERTIA | STATUS_ITERATION_CAP
 
 MAPPING_AUTO = 0
@@ -148,6 +149,14 @@ class Options(C.Structure):
         ("use_graph", C.c_int32),
         ("reserved", C.c_int32 * 3),
     ]
+
+
+class HistoryRecord(C.Structure):
+    _fields_ = [("system", C.c_int32), ("body", C.c_int32), ("time", C.c_double), ("time_step", C.c_double),
+                ("position", C.c_double * 3), ("spin", C.c_double * 3), ("velocity", C.c_double * 3),
+                ("mass", C.c_double), ("radius", C.c_double), ("radius_of_gyration_2", C.c_double),
+                ("love_number", C.c_double), ("scaled_dissipation_factor", C.c_double), ("lag_angle", C.c_double),
+                ("denergy_dt", C.c_double), ("migration_timescale", C.c_double)]
 
 
 class AdjointSeed(C.Structure):
@@ -263,6 +272,10 @@ def _declare(lib):
     lib.pos_gravity_acceleration.restype = C.c_int32
     lib.pos_additional_effects.argtypes = [C.c_void_p, _pd, _pd]
     lib.pos_additional_effects.restype = C.c_int32
+    lib.pos_history_enable.argtypes = [C.c_void_p, C.c_int32]
+    lib.pos_history_enable.restype = C.c_int32
+    lib.pos_history_drain.argtypes = [C.c_void_p, C.POINTER(HistoryRecord), C.c_int64, C.POINTER(C.c_int64)]
+    lib.pos_history_drain.restype = C.c_int32
     lib.pos_adjoint_record.argtypes = [C.c_void_p, C.c_int64, C.c_int32]
     lib.pos_adjoint_record.restype = C.c_int32
     lib.pos_adjoint.argtypes = [C.c_void_p, C.POINTER(AdjointSeed), C.POINTER(AdjointGrad)]
@@ -275,7 +288,8 @@ EXPORTED_SYMBOLS = (
     "pos_abi_version", "pos_last_error", "pos_create", "pos_step", "pos_synchronize",
     "pos_get_state", "pos_set_state", "pos_set_time_limit", "pos_set_historic_snapshot_period",
     "pos_get_counts", "pos_get_stream", "pos_get_options", "pos_tune", "pos_gravity_acceleration",
-    "pos_additional_effects", "pos_adjoint_record", "pos_adjoint", "pos_destroy",
+    "pos_additional_effects", "pos_history_enable", "pos_history_drain", "pos_adjoint_record", "pos_adjoint",
+    "pos_destroy",
 )
 
 

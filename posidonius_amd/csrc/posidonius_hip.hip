@@ -74,6 +74,7 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
     if (!valid) B.status |= POS_STATUS_COMPLETED;   // idle lanes never iterate
     B.tw = E.timestep_warning[s];
     B.left_index = E.left_index[b];
+    B.hcount = E.hist ? E.hist_count[s] : 0;
     const int nb = E.n_bodies;
     B.m = E.mass[b];
     B.inv_m = 1. / B.m;
@@ -133,6 +134,7 @@ __device__ inline void store_lane(const DevEnsemble &E, const WaveLds &lds, cons
         E.iteration[s] = B.iteration;
         E.status[s] = B.status;
         E.timestep_warning[s] = B.tw;
+        if (E.hist) E.hist_count[s] = B.hcount;
     }
 }
 
@@ -293,6 +295,8 @@ struct pos_ens {
     // per system / per body kinds in device order (adjoint support checks)
     std::vector<uint32_t> eff_dev;
     std::vector<int> coord_dev, evo_dev;
+    std::vector<double> mass_dev, sigma_dev, dt_dev;   // constants the history records carry
+    int hist_cap = 0;
     // discrete adjoint: checkpoints of the recorded run and the segment tape
     struct Adjoint {
         bool recorded = false;
@@ -300,7 +304,9 @@ struct pos_ens {
         int seg = 0, n_seg = 0;
         size_t state_bytes = 0;
         char *ckpt = nullptr;              // (n_seg + 1) states: segment starts, then the final state
-        double *tape = nullptr;            // (seg + 1) * TAPE_FIELDS * n_bodies
+        double *tape[2] = {nullptr, nullptr};   // two segment tapes, (seg + 1) * TAPE_FIELDS * n_bodies each
+        hipStream_t sweep = nullptr;            // stream of the reverse sweeps
+        hipEvent_t recorded_ev[2] = {nullptr, nullptr}, swept_ev[2] = {nullptr, nullptr};
         long long *iter0 = nullptr, *steps_done = nullptr;
         double *lx = nullptr, *lv = nullptr, *lL = nullptr, *gs = nullptr, *gk2 = nullptr, *gkf = nullptr;
         std::vector<void *> allocs;
@@ -338,6 +344,14 @@ int next_pow2(int n) {
     return g;
 }
 
+// Lanes of the group of a system of n bodies: the set group size (at least
+// n), exactly n (group_size < 0), or the next power of two (0).
+int group_lanes(int group_size, int n) {
+    if (group_size > 0) return std::max(group_size, n);
+    if (group_size < 0) return n;
+    return next_pow2(n);
+}
+
 // Build the wave table for the current mapping; systems are in device order
 // (sorted by body count).
 int build_launch(pos_ens *h, const pos_options &o) {
@@ -356,14 +370,14 @@ int build_launch(pos_ens *h, const pos_options &o) {
         int s = 0;
         while (s < ns) {
             const int n = h->n_per_sys_dev[s];
-            int G = o.group_size > 0 ? std::max(o.group_size, n) : next_pow2(n);
+            int G = group_lanes(o.group_size, n);
             if (G > 64) G = 64;
             const int per_wave = 64 / G;
             // systems with the same group size share waves
             int e = s;
             while (e < ns) {
                 const int ne = h->n_per_sys_dev[e];
-                const int Ge = o.group_size > 0 ? std::max(o.group_size, ne) : next_pow2(ne);
+                const int Ge = group_lanes(o.group_size, ne);
                 if (std::min(Ge, 64) != G) break;
                 ++e;
             }
@@ -470,6 +484,17 @@ double host_roche(double ma, double mb, double ra, double rb) {
     // universe.rs:186-193 (setup only, evaluated with the host libm)
     if (ma > mb) return (rb / 0.462) * std::pow(ma / mb, 1. / 3.);
     return (ra / 0.462) * std::pow(mb / ma, 1. / 3.);
+}
+
+void adjoint_release(pos_ens *h) {
+    pos_ens::Adjoint &A = h->adj;
+    for (void *p : A.allocs) (void)hipFree(p);
+    for (int i = 0; i < 2; ++i) {
+        if (A.recorded_ev[i]) (void)hipEventDestroy(A.recorded_ev[i]);
+        if (A.swept_ev[i]) (void)hipEventDestroy(A.swept_ev[i]);
+    }
+    if (A.sweep) (void)hipStreamDestroy(A.sweep);
+    A = pos_ens::Adjoint();
 }
 
 // The mutable device state of an ensemble (what a checkpoint must hold).
@@ -693,6 +718,12 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     h->eff_dev = eff;
     h->coord_dev = coord;
     h->evo_dev = evo_type;
+    h->mass_dev = m;
+    h->sigma_dev = sigma;
+    h->dt_dev = dt;
+    h->E.hist = nullptr;
+    h->E.hist_count = nullptr;
+    h->E.hist_cap = 0;
     *out = h;
     return POS_OK;
 }
@@ -985,7 +1016,7 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
     };
     int max_n = 0;
     for (int n : h->n_per_sys_dev) max_n = std::max(max_n, n);
-    std::vector<int> groups = {0};   // 0: next power of two per size class
+    std::vector<int> groups = {0, -1};   // next power of two / exactly N lanes, per size class
     for (int g = max_n; g <= 16; ++g)
         if (64 % g != 0) groups.push_back(g);   // exact packing for sizes that do not divide 64
     std::vector<int> ks = {16, 64};
@@ -1025,6 +1056,98 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
     return POS_OK;
 }
 
+int32_t pos_history_enable(pos_ens *h, int32_t capacity) {
+    if (!h) return fail(POS_E_INVALID, "handle is NULL");
+    if (capacity <= 0) return fail(POS_E_INVALID, "capacity must be positive");
+    if (h->E.hist) return fail(POS_E_INVALID, "historic snapshots are already enabled");
+    HIP_TRY(hipSetDevice(h->device));
+    const size_t bytes = (size_t)capacity * HIST_FIELDS * h->n_bodies * sizeof(double);
+    int rc;
+    double *buf = nullptr;
+    int *cnt = nullptr;
+    if ((rc = dalloc(h, &buf, (size_t)capacity * HIST_FIELDS * h->n_bodies)) || (rc = dalloc(h, &cnt, h->n_systems)))
+        return rc;
+    (void)bytes;
+    std::vector<int> zero(h->n_systems, 0);
+    if ((rc = upload(h, cnt, zero))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->E.hist = buf;
+    h->E.hist_count = cnt;
+    h->E.hist_cap = capacity;
+    h->hist_cap = capacity;
+    // captured launches hold the old kernel arguments
+    if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    h->graph_launches = 0;
+    return POS_OK;
+}
+
+int32_t pos_history_drain(pos_ens *h, pos_history_record *out, int64_t max_records, int64_t *n_out) {
+    if (!h) return fail(POS_E_INVALID, "handle is NULL");
+    if (!n_out) return fail(POS_E_INVALID, "n_out is NULL");
+    *n_out = 0;
+    if (!h->E.hist) return fail(POS_E_INVALID, "historic snapshots are not enabled");
+    HIP_TRY(hipSetDevice(h->device));
+    const int ns = h->n_systems, nb = h->n_bodies;
+    std::vector<int> cnt;
+    int rc;
+    if ((rc = download(h, cnt, h->E.hist_count, ns))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    int max_slot = 0;
+    int64_t total = 0;
+    for (int ds = 0; ds < ns; ++ds) {
+        const int c = std::min(cnt[ds], h->hist_cap);
+        max_slot = std::max(max_slot, c);
+        total += (int64_t)c * h->n_per_sys_dev[ds];
+    }
+    if (total > max_records) return fail(POS_E_LIMIT, "the output holds fewer records than are pending (" +
+                                                      std::to_string(total) + ")");
+    std::vector<double> buf;
+    if (max_slot > 0) {
+        if ((rc = download(h, buf, h->E.hist, (size_t)max_slot * HIST_FIELDS * nb))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    // input order: systems, then records, then bodies
+    std::vector<int> dev_of(ns);
+    for (int ds = 0; ds < ns; ++ds) dev_of[h->sys_perm[ds]] = ds;
+    int64_t k = 0;
+    for (int s = 0; s < ns; ++s) {
+        const int ds = dev_of[s];
+        const int c = std::min(cnt[ds], h->hist_cap);
+        const int b0 = h->body_offset_dev[ds], n = h->n_per_sys_dev[ds];
+        for (int r = 0; r < c; ++r) {
+            const double *rec = buf.data() + (size_t)r * HIST_FIELDS * nb;
+            for (int i = 0; i < n; ++i) {
+                const int db = b0 + i;
+                pos_history_record &o = out[k++];
+                auto f = [&](int field) { return rec[(size_t)field * nb + db]; };
+                o.system = s;
+                o.body = i;
+                o.time = f(HF_TIME);
+                o.time_step = h->dt_dev[ds];
+                for (int q = 0; q < 3; ++q) {
+                    o.position[q] = f(HF_X + q);
+                    o.spin[q] = f(HF_SPIN + q);
+                    o.velocity[q] = f(HF_V + q);
+                }
+                o.mass = h->mass_dev[db];
+                o.radius = f(HF_RADIUS);
+                o.radius_of_gyration_2 = f(HF_RG2);
+                o.love_number = f(HF_LOVE);
+                o.scaled_dissipation_factor = h->sigma_dev[db];
+                o.lag_angle = 0.;
+                o.denergy_dt = f(HF_DEDT);
+                o.migration_timescale = 0.;
+            }
+        }
+    }
+    std::vector<int> zero(ns, 0);
+    if ((rc = upload(h, h->E.hist_count, zero))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    *n_out = k;
+    return POS_OK;
+}
+
 int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
     if (!h) return fail(POS_E_INVALID, "handle is NULL");
     if (n_steps <= 0) return fail(POS_E_INVALID, "n_steps must be positive");
@@ -1041,8 +1164,8 @@ int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
     }
     HIP_TRY(hipSetDevice(h->device));
     pos_ens::Adjoint &A = h->adj;
-    for (void *p : A.allocs) (void)hipFree(p);
-    A = pos_ens::Adjoint();
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    adjoint_release(h);
     const int64_t seg64 = std::min<int64_t>(segment_steps > 0 ? segment_steps : 1000, n_steps);
     if (seg64 > (1 << 24)) return fail(POS_E_LIMIT, "segment_steps too large");
     const int seg = (int)seg64;
@@ -1055,7 +1178,7 @@ int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
     const size_t tape_bytes = (size_t)(seg + 1) * TAPE_FIELDS * nb * sizeof(double);
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    if (ckpt_bytes + tape_bytes + 64 * nb * sizeof(double) > free_b * 9 / 10)
+    if (ckpt_bytes + 2 * tape_bytes + 64 * nb * sizeof(double) > free_b * 9 / 10)
         return fail(POS_E_LIMIT, "checkpoints (" + std::to_string(ckpt_bytes >> 20) + " MiB) and segment tape (" +
                                  std::to_string(tape_bytes >> 20) + " MiB) exceed free device memory; use fewer steps "
                                  "per segment or fewer systems");
@@ -1065,13 +1188,19 @@ int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
         return POS_OK;
     };
     int rc;
-    if ((rc = aalloc((void **)&A.ckpt, ckpt_bytes)) || (rc = aalloc((void **)&A.tape, tape_bytes)) ||
+    if ((rc = aalloc((void **)&A.ckpt, ckpt_bytes)) || (rc = aalloc((void **)&A.tape[0], tape_bytes)) ||
+        (rc = aalloc((void **)&A.tape[1], tape_bytes)) ||
         (rc = aalloc((void **)&A.iter0, ns * sizeof(long long))) ||
         (rc = aalloc((void **)&A.steps_done, ns * sizeof(long long))) ||
         (rc = aalloc((void **)&A.lx, 3 * nb * sizeof(double))) || (rc = aalloc((void **)&A.lv, 3 * nb * sizeof(double))) ||
         (rc = aalloc((void **)&A.lL, 3 * nb * sizeof(double))) || (rc = aalloc((void **)&A.gs, nb * sizeof(double))) ||
         (rc = aalloc((void **)&A.gk2, nb * sizeof(double))) || (rc = aalloc((void **)&A.gkf, nb * sizeof(double))))
         return rc;
+    HIP_TRY(hipStreamCreateWithFlags(&A.sweep, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+        HIP_TRY(hipEventCreateWithFlags(&A.recorded_ev[i], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&A.swept_ev[i], hipEventDisableTiming));
+    }
     A.seg = seg;
     A.n_seg = (int)n_seg;
     A.n_steps = n_steps;
@@ -1120,21 +1249,41 @@ int32_t pos_adjoint(pos_ens *h, const pos_adjoint_seed *seed, pos_adjoint_grad *
     }
     AdjArgs Ar{};
     Ar.E = h->E;
-    Ar.T = TapeRef{A.tape, nb};
     Ar.steps_done = A.steps_done;
     Ar.lx = A.lx; Ar.lv = A.lv; Ar.lL = A.lL;
     Ar.gs = A.gs; Ar.gk2 = A.gk2; Ar.gkf = A.gkf;
+    // Segment s is recomputed from its checkpoint on the handle's stream
+    // (writing tape s % 2) while segment s + 1 is swept in reverse on the
+    // sweep stream: the sweep of s waits for the recomputation of s, and the
+    // recomputation of s - 2 waits for the sweep of s, which frees its tape.
+    auto seg_len = [&](int s) { return (int)std::min<int64_t>(A.seg, A.n_steps - (int64_t)s * A.seg); };
+    auto recompute = [&](int s) -> int {
+        const int buf = s & 1;
+        if (s + 2 < A.n_seg) HIP_TRY(hipStreamWaitEvent(h->stream, A.swept_ev[buf], 0));
+        int r = adjoint_copy_state(h, A.ckpt + (size_t)s * A.state_bytes, false);
+        if (r) return r;
+        hipLaunchKernelGGL(whfast_record_kernel, dim3(h->grid), dim3(64), 0, h->stream, h->E, seg_len(s),
+                           TapeRef{A.tape[buf], nb});
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(A.recorded_ev[buf], h->stream));
+        return POS_OK;
+    };
+    // the sweep stream must also see the seeds uploaded on the handle's stream
+    HIP_TRY(hipEventRecord(A.swept_ev[0], h->stream));
+    HIP_TRY(hipStreamWaitEvent(A.sweep, A.swept_ev[0], 0));
+    if ((rc = recompute(A.n_seg - 1))) return rc;
     for (int s = A.n_seg - 1; s >= 0; --s) {
-        const int64_t start = (int64_t)s * A.seg;
-        const int len = (int)std::min<int64_t>(A.seg, A.n_steps - start);
-        if ((rc = adjoint_copy_state(h, A.ckpt + (size_t)s * A.state_bytes, false))) return rc;
-        hipLaunchKernelGGL(whfast_record_kernel, dim3(h->grid), dim3(64), 0, h->stream, h->E, len, Ar.T);
+        const int buf = s & 1;
+        HIP_TRY(hipStreamWaitEvent(A.sweep, A.recorded_ev[buf], 0));
+        Ar.T = TapeRef{A.tape[buf], nb};
+        Ar.seg_start = (int64_t)s * A.seg;
+        Ar.seg_len = seg_len(s);
+        hipLaunchKernelGGL(adjoint_segment_kernel, dim3(h->grid), dim3(64), 0, A.sweep, Ar);
         HIP_TRY(hipGetLastError());
-        Ar.seg_start = start;
-        Ar.seg_len = len;
-        hipLaunchKernelGGL(adjoint_segment_kernel, dim3(h->grid), dim3(64), 0, h->stream, Ar);
-        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(A.swept_ev[buf], A.sweep));
+        if (s > 0 && (rc = recompute(s - 1))) return rc;
     }
+    HIP_TRY(hipStreamWaitEvent(h->stream, A.swept_ev[0], 0));
     if ((rc = adjoint_copy_state(h, A.ckpt + (size_t)A.n_seg * A.state_bytes, false))) return rc;
     // results, device -> input order
     std::vector<double> lx, lv, lL, gs, gk2, gkf;
@@ -1163,8 +1312,9 @@ void pos_destroy(pos_ens *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
+    if (h->adj.sweep) (void)hipStreamSynchronize(h->adj.sweep);
+    adjoint_release(h);
     for (void *p : h->allocs) (void)hipFree(p);
-    for (void *p : h->adj.allocs) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }

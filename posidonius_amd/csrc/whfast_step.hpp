@@ -107,6 +107,19 @@ struct DevEnsemble {
     // launch table: wave w covers device systems wave_first[w] .. +wave_count[w]-1
     const int *wave_first, *wave_count, *wave_group;
     int n_waves;
+    // historic snapshots (optional): record r of system s is written at
+    // hist[(r * HIST_FIELDS + f) * n_bodies + body], hist_count[s] records so far
+    double *hist;
+    int *hist_count;
+    int hist_cap;
+};
+
+// Fields of one body's historic-snapshot record on the device (the host adds
+// the constant ones: time step, mass, dissipation factor, lag angle and
+// migration timescale, src/integrator/output.rs:90-134).
+enum HistField {
+    HF_TIME = 0, HF_X = 1, HF_SPIN = 4, HF_V = 7, HF_RADIUS = 10, HF_RG2 = 11, HF_LOVE = 12, HF_DEDT = 13,
+    HIST_FIELDS = 14
 };
 
 // ---- LDS of one wavefront ---------------------------------------------------
@@ -131,7 +144,7 @@ enum : uint32_t {
 struct Lane {
     int lane, gbase, k, n, host, coord, sys, body;
     uint32_t flags, status;
-    int tw, evo, left_index;
+    int tw, evo, left_index, hcount;
     long long iteration;
     double t, dt, last_hist;
     double m, inv_m;
@@ -1116,6 +1129,69 @@ __device__ inline void merge_status(WaveLds &lds, Lane &B) {
 }
 
 // ===========================================================================
+// Historic snapshot record (whfast.rs:240-256, output.rs:90-134): the state
+// after the snapshot branch's spin and evolution refresh.  denergy_dt
+// (tides/common.rs:263-281, Bolmont et al. 2015 eq. 32) is evaluated at this
+// state: the reference evaluates it from the tidal internals of the last
+// effects evaluation of the previous step, half a step earlier.
+// ===========================================================================
+__device__ inline void write_history_record(const DevEnsemble &E, WaveLds &lds, Lane &B) {
+    const int lane = B.lane;
+    if (B.valid()) {
+        lds.xb[0][lane] = B.x[0]; lds.xb[1][lane] = B.x[1]; lds.xb[2][lane] = B.x[2];
+        lds.xb[3][lane] = B.v[0]; lds.xb[4][lane] = B.v[1]; lds.xb[5][lane] = B.v[2];
+    }
+    wsync();
+    double dedt = 0.;
+    if (B.orbiting() && B.tides()) {
+        const int hl = B.hl();
+        double r[3], u[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) { r[q] = B.x[q] - lds.xb[q][hl]; u[q] = B.v[q] - lds.xb[3 + q][hl]; }
+        const double d = sqrt(p2(r[0]) + p2(r[1]) + p2(r[2]));
+        const double vr = (r[0] * u[0] + r[1] * u[1] + r[2] * u[2]) / d;
+        const double hm = lds.m[hl];
+        const double R = lds.R[lane];
+        const double op = 4.5 * p2(hm) * p10(R) * lds.sigma[lane] / p7(d);
+        const double diss_pm = -13.5 * vr / p8(d) * (p2(hm) * p10(R) * lds.sigma[lane]);
+        const double f2 = op / d;
+        const double sp[3] = {lds.s[0][lane], lds.s[1][lane], lds.s[2][lane]};
+        const double rs = r[0] * sp[0] + r[1] * sp[1] + r[2] * sp[2];
+        const double c[3] = {r[1] * u[2] - r[2] * u[1], r[2] * u[0] - r[0] * u[2], r[0] * u[1] - r[1] * u[0]};
+        double torque_dot_spin = 0.;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) torque_dot_spin += -op * (d * sp[q] - rs * r[q] / d - c[q] / d) * sp[q];
+        const double ru = r[0] * u[0] + r[1] * u[1] + r[2] * u[2];
+        const double w0 = sp[1] * r[2] - sp[2] * r[1] - u[0];
+        const double w1 = sp[2] * r[0] - sp[0] * r[2] - u[1];
+        const double w2 = sp[0] * r[1] - sp[1] * r[0] - u[2];
+        dedt = -((1.0 / d * (diss_pm + f2 * vr)) * ru + f2 * (w0 * u[0] + w1 * u[1] + w2 * u[2])) - torque_dot_spin;
+    }
+    wsync();
+    const int slot = B.hcount;
+    if (slot >= E.hist_cap) {
+        B.status |= POS_STATUS_HISTORY_OVERFLOW;
+        return;
+    }
+    if (B.valid()) {
+        const size_t nb = (size_t)E.n_bodies;
+        double *rec = E.hist + (size_t)slot * HIST_FIELDS * nb + B.body;
+        rec[HF_TIME * nb] = B.t;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            rec[(HF_X + q) * nb] = B.x[q];
+            rec[(HF_SPIN + q) * nb] = lds.s[q][lane];
+            rec[(HF_V + q) * nb] = B.v[q];
+        }
+        rec[HF_RADIUS * nb] = lds.R[lane];
+        rec[HF_RG2 * nb] = E.rg2[B.body];
+        rec[HF_LOVE * nb] = lds.k2[lane];
+        rec[HF_DEDT * nb] = dedt;
+    }
+    B.hcount = slot + 1;
+}
+
+// ===========================================================================
 // Step tape: what the reverse sweep of the adjoint needs from each step,
 // written by the recording variant of the step.  Row r of a segment holds,
 // per body (component-major, body index fastest), the state at the start of
@@ -1152,6 +1228,7 @@ __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, cons
             if (B.flags & F_EVOLUTION) evolve_body(E, lds, B, B.t);
             own_spin(lds, B);
             wsync();
+            if (E.hist) write_history_record(E, lds, B);
             B.last_hist = first ? 0. : B.last_hist + period;
         }
     }

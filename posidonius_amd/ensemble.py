@@ -112,6 +112,29 @@ class Ensemble:
         abi.check(self._lib.pos_additional_effects(self._h, abi.ptr(acc, C.c_double), abi.ptr(dl, C.c_double)))
         return acc, dl
 
+    # -- historic snapshots ---------------------------------------------------
+    def enable_history(self, capacity: int):
+        """Keep up to `capacity` historic-snapshot records per system on the
+        device between drains."""
+        abi.check(self._lib.pos_history_enable(self._h, int(capacity)))
+        self._hist_cap = int(capacity)
+
+    def drain_history(self) -> np.ndarray:
+        """Pending historic-snapshot records (structured array, input order:
+        system, record, body); clears the device buffer."""
+        cap = getattr(self, "_hist_cap", 0)
+        n_max = cap * self.n_bodies
+        buf = (abi.HistoryRecord * max(n_max, 1))()
+        n = C.c_int64()
+        abi.check(self._lib.pos_history_drain(self._h, buf, n_max, C.byref(n)))
+        dt = np.dtype([("system", np.int32), ("body", np.int32), ("time", np.float64), ("time_step", np.float64),
+                       ("position", np.float64, 3), ("spin", np.float64, 3), ("velocity", np.float64, 3),
+                       ("mass", np.float64), ("radius", np.float64), ("radius_of_gyration_2", np.float64),
+                       ("love_number", np.float64), ("scaled_dissipation_factor", np.float64),
+                       ("lag_angle", np.float64), ("denergy_dt", np.float64), ("migration_timescale", np.float64)])
+        assert dt.itemsize == C.sizeof(abi.HistoryRecord)
+        return np.frombuffer(bytes(buf)[:n.value * dt.itemsize], dtype=dt).copy()
+
     # -- discrete adjoint -----------------------------------------------------
     def adjoint_record(self, n_steps: int, segment_steps: int = 0):
         """Advance n_steps like ``step`` and keep a checkpoint every

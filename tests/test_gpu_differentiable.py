@@ -1,0 +1,87 @@
+"""Gradients of the differentiable form against central finite differences
+of the oracle (the restated reference integrator).
+
+Loss per system: a random combination of every planet's final semi-major
+axis, eccentricity and spin rate after n steps.  Gradients with respect to
+each planet's initial a, e, log dissipation factor and Love number k2 (tidal
+and fluid tied) come from torch autograd through WHFastSteps (the GPU
+adjoint); the finite differences perturb one parameter of one planet slot in
+every system at once (systems are independent) by h = 1e-6 relative and run
+the oracle.  The bar is 1e-6 relative, above the finite differences' own
+rounding noise (30 eps |loss| / h), which dominates for the weak tidal
+sensitivities at the default dissipation.
+"""
+import numpy as np
+import pytest
+import torch
+
+from posidonius_amd import differentiable as dF, generators as gen
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-6
+
+
+def losses(spec, out, w, ref):
+    """Per-system loss from final a, e, spin rate (torch tensors)."""
+    per_body = w[:, 0] * out["a"] / ref["a"] + w[:, 1] * out["e"] + w[:, 2] * out["spin_rate"] / ref["w"]
+    sidx = torch.as_tensor(np.repeat(np.arange(spec.n_systems), spec.n_per_system()))
+    return torch.zeros(spec.n_systems, dtype=torch.float64).index_add_(0, sidx, per_body)
+
+
+def oracle_losses(oracle, spec, params, n_steps, w, ref):
+    with torch.no_grad():
+        x0, v0 = dF.state_from_elements(spec, params["a"], params["e"])
+        s = spec.copy()
+        s.body["position"] = x0.numpy().copy()
+        s.body["velocity"] = v0.numpy().copy()
+        s.body["tides_dissipation_factor"] = np.exp(params["log_sigma"].numpy())
+        s.body["tides_dissipation_factor_scale"] = np.ones(spec.n_bodies)
+        s.body["tides_love_number"] = params["k2"].numpy().copy()
+        s.body["flattening_love_number"] = params["k2"].numpy().copy()
+        st = oracle.run(s, n_steps, threads=8)
+        x, v, L = (torch.tensor(st[k]) for k in ("position", "velocity", "angular_momentum"))
+        a, e = dF.elements_from_state(spec, x, v)
+        out = {"a": a, "e": e, "spin_rate": dF.spin_rate(spec, L)}
+        return losses(spec, out, w, ref).numpy()
+
+
+@pytest.mark.parametrize("sigma_boost", [1.0, 300.0])
+def test_gradients_match_oracle_finite_differences(oracle, sigma_boost):
+    n_steps = 600
+    spec = gen.config4(16)
+    spec.body["tides_dissipation_factor_scale"] = spec.body["tides_dissipation_factor_scale"] * sigma_boost
+    rng = np.random.default_rng(23)
+    w = torch.tensor(rng.standard_normal((spec.n_bodies, 3)))
+    planet = spec.elements["a"] > 0
+    w[~torch.tensor(planet)] = 0.0
+    ref = {"a": torch.tensor(np.where(planet, spec.elements["a"], 1.0)),
+           "w": torch.tensor(np.linalg.norm(spec.body["spin"], axis=1))}
+    params = dF.parameters(spec)
+    out = dF.integrate(spec, n_steps, params, segment_steps=256)
+    loss = losses(spec, out, w, ref)
+    loss.sum().backward()
+    base = {k: v.detach().clone() for k, v in params.items()}
+    l0 = oracle_losses(oracle, spec, base, n_steps, w, ref)
+    np.testing.assert_allclose(loss.detach().numpy(), l0, rtol=1e-13)
+    hosts = spec.body_offset[:-1]
+    worst = {}
+    for key in ("a", "e", "log_sigma", "k2"):
+        for slot in range(1, 4):
+            idx = hosts + slot
+            step = H * np.abs(base[key].numpy()[idx]) if key != "log_sigma" else np.full(len(idx), H)
+            lp = {k: v.clone() for k, v in base.items()}
+            lm = {k: v.clone() for k, v in base.items()}
+            lp[key][idx] += torch.tensor(step)
+            lm[key][idx] -= torch.tensor(step)
+            fd = (oracle_losses(oracle, spec, lp, n_steps, w, ref) - oracle_losses(oracle, spec, lm, n_steps, w, ref)) / (2 * step)
+            adj = params[key].grad.numpy()[idx]
+            noise = 30 * np.finfo(float).eps * np.abs(l0) / step
+            err = np.abs(fd - adj)
+            ok = err <= 1e-6 * np.abs(adj) + noise
+            rel = err / np.maximum(np.abs(adj), 1e-300)
+            print(f"boost {sigma_boost:g} d/d{key}[planet {slot}]: |adj| ~ {np.median(np.abs(adj)):.3e}, "
+                  f"max rel err {rel.max():.2e}, noise/|adj| {np.median(noise / np.maximum(np.abs(adj), 1e-300)):.1e}",
+                  flush=True)
+            worst[(key, slot)] = rel.max()
+            assert ok.all(), (key, slot, fd[~ok][:3], adj[~ok][:3], noise[~ok][:3])

@@ -1,0 +1,65 @@
+"""Historic snapshots written on the device (output.rs:90-134 records, taken
+where WHFast::iterate takes them, whfast.rs:240-256)."""
+import numpy as np
+import pytest
+
+from posidonius_amd import abi, generators as gen
+from tests.oracle_runner import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def snapshot_steps(t0, dt, period, n_steps, last=-1.0):
+    """Steps (counted from the start) at whose beginning the reference takes
+    a historic snapshot, replaying its time accumulation."""
+    out, t = [], t0
+    for k in range(n_steps):
+        if last < 0.0 or last + period <= t:
+            out.append(k)
+            last = 0.0 if last < 0.0 else last + period
+        t = t + dt
+    return out
+
+
+def test_history_records_match_the_oracle_state(oracle):
+    from posidonius_amd.ensemble import Ensemble
+    spec, _ = load_golden("test_integrator-whfast_democraticheliocentric")
+    spec.system["historic_snapshot_period"][:] = 7.3 * spec.system["time_step"]
+    n = 120
+    with Ensemble(spec, mapping="packed", steps_per_launch=16) as ens:
+        ens.enable_history(64)
+        ens.step(n)
+        rec = ens.drain_history()
+        st = ens.get_state(fields=("status",))
+    assert not (st["status"][0] & abi.STATUS_HISTORY_OVERFLOW)
+    steps = snapshot_steps(0.0, spec.system["time_step"][0], spec.system["historic_snapshot_period"][0], n)
+    nb = spec.n_bodies
+    assert len(rec) == len(steps) * nb
+    I = spec.body["moment_of_inertia"]
+    for r, k in enumerate(steps[:6]):
+        ref = oracle.run(spec, k) if k > 0 else None
+        got = rec[r * nb:(r + 1) * nb]
+        x = spec.body["position"] if ref is None else ref["position"]
+        v = spec.body["velocity"] if ref is None else ref["velocity"]
+        L = spec.body["angular_momentum"] if ref is None else ref["angular_momentum"]
+        np.testing.assert_array_equal(got["position"], x)
+        np.testing.assert_array_equal(got["velocity"], v)
+        np.testing.assert_allclose(got["spin"], L / I[:, None], rtol=1e-15)
+        assert np.all(got["body"] == np.arange(nb))
+        np.testing.assert_allclose(got["time"], k * spec.system["time_step"][0], rtol=1e-12)
+    planets = rec["body"] > 0
+    assert np.all(rec["denergy_dt"][planets] != 0) and np.all(rec["denergy_dt"][~planets] == 0)
+
+
+def test_history_overflow_is_flagged_not_fatal():
+    from posidonius_amd.ensemble import Ensemble
+    spec = gen.config1(8)
+    spec.system["historic_snapshot_period"][:] = spec.system["time_step"]
+    with Ensemble(spec, steps_per_launch=8) as ens:
+        ens.enable_history(4)
+        ens.step(20)
+        st = ens.get_state(fields=("status", "current_iteration"))
+        rec = ens.drain_history()
+    assert np.all(st["status"] & abi.STATUS_HISTORY_OVERFLOW)
+    assert np.all(st["current_iteration"] == 20)
+    assert len(rec) == 4 * spec.n_bodies

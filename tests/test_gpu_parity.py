@@ -92,10 +92,10 @@ def test_configs_bitwise_against_oracle(Ensemble, oracle, cfg, n, steps):
     assert_same_state(out, ref, where=f"config {cfg}")
 
 
-@pytest.mark.parametrize("group_size,K,graph", [(0, 1, False), (0, 7, False), (8, 64, True), (16, 32, False), (3, 16, True)])
+@pytest.mark.parametrize("group_size,K,graph", [(0, 1, False), (0, 7, False), (8, 64, True), (16, 32, False), (3, 16, True), (-1, 64, True)])
 def test_launch_mappings_agree(Ensemble, oracle, group_size, K, graph):
     """Group size, steps per launch and graph replay do not change a bit."""
-    spec = gen.config1(96) if group_size != 16 else gen.config2(96)
+    spec = gen.config2(96) if group_size in (16, -1) else gen.config1(96)
     if group_size == 3:
         spec = gen.config1(96)
     steps = 100
