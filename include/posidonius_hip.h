@@ -182,6 +182,8 @@ typedef struct pos_state {
     double   *last_historic_snapshot_time; /* [n_systems]                     */
     int64_t  *current_iteration;     /* [n_systems]                           */
     uint32_t *status;                /* [n_systems] POS_STATUS_* bits         */
+    int32_t  *evolver_left_index;    /* [nb] Evolver.left_index (evolution.rs:26) */
+    int32_t  *timestep_warning;      /* [n_systems] WHFast.timestep_warning   */
 } pos_state;
 
 /* Launch mapping chosen by the autotuner or by the caller. */

@@ -36,7 +36,9 @@ template <class R> void store_state(const oracle::System<R> &S, int b0, int s, p
         if (o->radius) o->radius[b] = val(P.radius);
         if (o->radius_of_gyration_2) o->radius_of_gyration_2[b] = val(P.rg2);
         if (o->moment_of_inertia) o->moment_of_inertia[b] = val(P.inertia);
+        if (o->evolver_left_index) o->evolver_left_index[b] = (int32_t)S.ev[i].left_index;
     }
+    if (o->timestep_warning) o->timestep_warning[s] = S.timestep_warning;
     if (o->current_time) o->current_time[s] = val(S.t);
     if (o->last_historic_snapshot_time) o->last_historic_snapshot_time[s] = S.last_hist;
     if (o->current_iteration) o->current_iteration[s] = S.iteration;

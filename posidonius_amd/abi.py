@@ -137,6 +137,8 @@ class State(C.Structure):
         ("last_historic_snapshot_time", _pd),
         ("current_iteration", _pi64),
         ("status", _pu32),
+        ("evolver_left_index", _pi32),
+        ("timestep_warning", _pi32),
     ]
 
 
@@ -192,9 +194,11 @@ STATE_FIELDS = {
     "last_historic_snapshot_time": (0, np.float64),
     "current_iteration": (0, np.int64),
     "status": (0, np.uint32),
+    "evolver_left_index": (1, np.int32),
+    "timestep_warning": (0, np.int32),
 }
 
-_CTYPE = {np.float64: C.c_double, np.int64: C.c_int64, np.uint32: C.c_uint32}
+_CTYPE = {np.float64: C.c_double, np.int64: C.c_int64, np.uint32: C.c_uint32, np.int32: C.c_int32}
 
 
 def alloc_state(n_systems: int, n_bodies: int, fields=None):

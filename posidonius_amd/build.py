@@ -47,5 +47,23 @@ def build_library(verbose: bool = False, extra_flags=None) -> str:
     return OUT
 
 
+CLI_SRC = os.path.join(HERE, "host", "posidonius_cli.cpp")
+CLI_OUT = os.path.join(HERE, "posidonius-hip")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """The `posidonius-hip start|resume` host, linked against the library."""
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(hipcc())))
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"), CLI_SRC,
+           "-o", CLI_OUT + ".tmp", "-L", HERE, "-lposidonius_hip", "-Wl,-rpath,$ORIGIN",
+           "-L", os.path.join(rocm, "lib"), "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(CLI_OUT + ".tmp", CLI_OUT)
+    return CLI_OUT
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True, extra_flags=sys.argv[1:]))
+    print(build_cli(verbose=True))

@@ -815,6 +815,18 @@ int32_t pos_get_state(pos_ens *h, pos_state *o) {
         HIP_TRY(hipStreamSynchronize(h->stream));
         for (int ds = 0; ds < ns; ++ds) o->status[h->sys_perm[ds]] = st[ds];
     }
+    if (o->evolver_left_index) {
+        std::vector<int> li;
+        if ((rc = download(h, li, E.left_index, nb))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (int db = 0; db < nb; ++db) o->evolver_left_index[h->body_perm[db]] = li[db];
+    }
+    if (o->timestep_warning) {
+        std::vector<int> tw;
+        if ((rc = download(h, tw, E.timestep_warning, ns))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (int ds = 0; ds < ns; ++ds) o->timestep_warning[h->sys_perm[ds]] = tw[ds];
+    }
     return POS_OK;
 }
 
@@ -871,6 +883,18 @@ int32_t pos_set_state(pos_ens *h, const pos_state *in) {
         std::vector<uint32_t> st(ns);
         for (int ds = 0; ds < ns; ++ds) st[ds] = in->status[h->sys_perm[ds]];
         if ((rc = upload(h, E.status, st))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    if (in->evolver_left_index) {
+        std::vector<int> li(nb);
+        for (int db = 0; db < nb; ++db) li[db] = in->evolver_left_index[h->body_perm[db]];
+        if ((rc = upload(h, E.left_index, li))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    if (in->timestep_warning) {
+        std::vector<int> tw(ns);
+        for (int ds = 0; ds < ns; ++ds) tw[ds] = in->timestep_warning[h->sys_perm[ds]];
+        if ((rc = upload(h, E.timestep_warning, tw))) return rc;
         HIP_TRY(hipStreamSynchronize(h->stream));
     }
     return POS_OK;

@@ -1,0 +1,72 @@
+"""`posidonius-hip start|resume` end to end on the GPU.
+
+A run to completion and a run that stops early (shorter time limit) and is
+resumed from its last recovery snapshot with the full time limit must write
+byte-identical history files (the resume truncates the history to what the
+snapshot accounts for, output.rs:64-87, and continues bit for bit).  The
+history records follow the reference's 156-byte layout (output.rs:90-134).
+"""
+import json
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.oracle_runner import GOLDEN_DIR
+
+pytestmark = pytest.mark.gpu
+
+CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "posidonius_amd", "posidonius-hip")
+RECORD = struct.Struct("<ddi12d5d")
+
+
+def read_history(path, n):
+    data = open(path, "rb").read()
+    assert len(data) % (RECORD.size * n) == 0
+    return [RECORD.unpack_from(data, o) for o in range(0, len(data), RECORD.size)]
+
+
+def write_case(path, hist_every, rec_every, time_limit=None):
+    with open(os.path.join(GOLDEN_DIR, "test_integrator-whfast_democraticheliocentric.json")) as f:
+        case = json.load(f)["case"]
+    dt = case["time_step"]
+    case["historic_snapshot_period"] = hist_every * dt
+    case["recovery_snapshot_period"] = rec_every * dt
+    if time_limit is not None:
+        case["universe"]["time_limit"] = time_limit
+    with open(path, "w") as f:
+        json.dump(case, f)
+    return case
+
+
+def cli(*args):
+    r = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def test_start_and_resume_write_identical_history(tmp_path):
+    full = write_case(tmp_path / "case.json", 10, 50)
+    n = full["universe"]["n_particles"]
+    cli("start", str(tmp_path / "case.json"), str(tmp_path / "a.json"), str(tmp_path / "a.bin"), "-s")
+    rec_a = read_history(tmp_path / "a.bin", n)
+    # first snapshot: the initial state, ids and times in order
+    first = rec_a[:n]
+    for i, r in enumerate(first):
+        assert r[0] == 0.0 and r[2] == i
+        p = full["universe"]["particles"][i]["inertial_position"]
+        assert r[3:6] == (p["x"], p["y"], p["z"])
+    times = [r[0] for r in rec_a[::n]]
+    assert times == sorted(times) and len(times) >= 19
+    # stop early, then resume with the full time limit
+    write_case(tmp_path / "short.json", 10, 50, time_limit=full["universe"]["time_limit"] * 0.6)
+    cli("start", str(tmp_path / "short.json"), str(tmp_path / "b.json"), str(tmp_path / "b.bin"), "-s")
+    assert os.path.getsize(tmp_path / "b.bin") < os.path.getsize(tmp_path / "a.bin")
+    snap = json.load(open(tmp_path / "b.json"))
+    assert snap["current_iteration"] > 0 and snap["n_historic_snapshots"] > 0
+    cli("resume", str(tmp_path / "b.json"), str(tmp_path / "b.bin"), "-s", "--time-limit",
+        repr(full["universe"]["time_limit"]))
+    assert open(tmp_path / "b.bin", "rb").read() == open(tmp_path / "a.bin", "rb").read()

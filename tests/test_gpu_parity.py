@@ -22,7 +22,7 @@ UNSUPPORTED = {"test_disk-enabled_disk", "test_general_relativity-anderson1975",
 SUPPORTED = [n for n in golden_names() if n not in UNSUPPORTED]
 STATE_KEYS = ("position", "velocity", "acceleration", "angular_momentum", "spin", "velocity_error",
               "angular_momentum_error", "radius", "moment_of_inertia", "current_time", "current_iteration",
-              "status")
+              "status", "evolver_left_index", "timestep_warning")
 
 
 def log(msg):
