@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -390,6 +391,11 @@ int build_launch(pos_ens *h, const pos_options &o) {
         }
         h->steps_per_launch = o.steps_per_launch > 0 ? o.steps_per_launch : 64;
     }
+    // Heaviest wavefronts (largest systems) first: workgroups are dispatched
+    // in index order, so the light ones fill the tail of the launch.
+    std::reverse(h->wave_first.begin(), h->wave_first.end());
+    std::reverse(h->wave_count.begin(), h->wave_count.end());
+    std::reverse(h->wave_group.begin(), h->wave_group.end());
     const int nw = (int)h->wave_first.size();
     int rc;
     if (!h->d_wave_first) {
