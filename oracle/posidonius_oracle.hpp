@@ -222,7 +222,7 @@ inline bool load_system(const pos_ensemble_desc *d, int s, System<R> &S, std::st
     if (eff & POS_EFFECT_DISK) { if (err) *err = "disk effect not supported"; return false; }
     S.coord = d->coordinates[s];
     S.gr_impl = d->gr_implementation[s];
-    if (S.gr && S.gr_impl != POS_GR_KIDDER1995) { if (err) *err = "only Kidder1995 GR is supported"; return false; }
+    if (S.gr && S.gr_impl == POS_GR_DISABLED) S.gr = false;
     S.dt = R(d->time_step[s]);
     S.half_dt = R(0.5) * S.dt;
     S.time_limit = R(d->time_limit[s]);
@@ -380,6 +380,11 @@ template <class R> inline void inertial_to_heliocentric(System<R> &S) {
     H.hdist = R(0.); H.hvr = R(0.); H.hnv2 = R(0.); H.hnv = R(0.);
 }
 
+// Gravity terms a WHFast splitting leaves out of gravity_calculate_acceleration
+// (universe.rs:66-70): Jacobi drops the host - first body pair, the
+// heliocentric splittings every host pair.
+enum IgnoreTerms { IGNORE_NONE = 0, IGNORE_WHFAST_ONE = 1, IGNORE_WHFAST_TWO = 2 };
+
 // ---------------------------------------------------------------------------
 // Effect kernels
 // ---------------------------------------------------------------------------
@@ -411,6 +416,7 @@ template <class R> inline void flattening_initialize(System<R> &S) {
 }
 // general_relativity.rs:86-109
 template <class R> inline void gr_initialize(System<R> &S) {
+    if (S.gr_impl == POS_GR_NEWHALL1983 || S.gr_impl == POS_GR_DISABLED) return;
     Body<R> &H = S.p[S.host];
     H.gr_factor = R(0.);
     H.gr_acc = v3zero<R>(); H.gr_dLdt = v3zero<R>();
@@ -723,6 +729,208 @@ template <class R> inline void kidder_spin_orbit(System<R> &S) {
 }
 
 // wind.rs:61-91
+// general_relativity.rs:641-681: the gravitational accelerations of the last
+// gravity evaluation plus the host pair terms the splitting ignored.  The
+// pair separation mixes the host's inertial position with the body's
+// heliocentric one, as the reference does.  Order: host, then the other
+// bodies in index order.
+template <class R> inline void gr_newtonian_accelerations(const System<R> &S, V3<R> acc[]) {
+    const Body<R> &H = S.p[S.host];
+    acc[0] = H.a_grav;
+    int c = 1;
+    for (int i = 0; i < S.n; ++i)
+        if (i != S.host) acc[c++] = S.p[i].a_grav;
+    const int ignore = S.coord == POS_COORD_JACOBI ? IGNORE_WHFAST_ONE : IGNORE_WHFAST_TWO;
+    const int np = ignore == IGNORE_WHFAST_ONE ? 1 : S.n - 1;
+    c = 1;
+    for (int i = 0; i < S.n && c <= np; ++i) {
+        if (i == S.host) continue;
+        const Body<R> &P = S.p[i];
+        const R dx = H.x.x - P.hx.x, dy = H.x.y - P.hx.y, dz = H.x.z - P.hx.z;
+        const R r2 = pw2(dx) + pw2(dy) + pw2(dz);
+        const R r = sqrt_r(r2);
+        const R prefac = R(G) / (r2 * r);
+        const R pms = prefac * H.m, pmp = prefac * P.m;
+        acc[0].x = acc[0].x - pmp * dx; acc[0].y = acc[0].y - pmp * dy; acc[0].z = acc[0].z - pmp * dz;
+        acc[c].x = acc[c].x + pms * dx; acc[c].y = acc[c].y + pms * dy; acc[c].z = acc[c].z + pms * dz;
+        ++c;
+    }
+}
+
+// Anderson et al. 1975 via Jacobi coordinates (general_relativity.rs:461-639)
+template <class R> inline void anderson1975(System<R> &S) {
+    Body<R> &H = S.p[S.host];
+    V3<R> an[MAXB];
+    gr_newtonian_accelerations(S, an);
+    V3<R> jx[MAXB], jv[MAXB], ja[MAXB];
+    int idx[MAXB], np = 0;
+    R eta = H.m;
+    V3<R> s{eta * H.x.x, eta * H.x.y, eta * H.x.z};
+    V3<R> sv{eta * H.v.x, eta * H.v.y, eta * H.v.z};
+    V3<R> sa{eta * an[0].x, eta * an[0].y, eta * an[0].z};
+    int c = 0;
+    for (int i = 0; i < S.n; ++i) {
+        if (i == S.host) continue;
+        ++c;
+        const Body<R> &P = S.p[i];
+        if (!(P.roles & POS_ROLE_GR_ORBITING)) continue;
+        const R ei = R(1.) / eta;
+        eta = eta + P.m;
+        const R pme = eta * ei;
+        V3<R> x{P.x.x - s.x * ei, P.x.y - s.y * ei, P.x.z - s.z * ei};
+        V3<R> v{P.v.x - sv.x * ei, P.v.y - sv.y * ei, P.v.z - sv.z * ei};
+        V3<R> a{an[c].x - sa.x * ei, an[c].y - sa.y * ei, an[c].z - sa.z * ei};
+        s.x = s.x * pme + P.m * x.x; s.y = s.y * pme + P.m * x.y; s.z = s.z * pme + P.m * x.z;
+        sv.x = sv.x * pme + P.m * v.x; sv.y = sv.y * pme + P.m * v.y; sv.z = sv.z * pme + P.m * v.z;
+        sa.x = sa.x * pme + P.m * a.x; sa.y = sa.y * pme + P.m * a.y; sa.z = sa.z * pme + P.m * a.z;
+        jx[np] = x; jv[np] = v; ja[np] = a; idx[np] = i; ++np;
+    }
+    const R mtot = eta;
+    const R mu = H.mg;
+    const R c2 = R(SPEED_OF_LIGHT_2);
+    for (int q = 0; q < np; ++q) {
+        V3<R> vi = jv[q];
+        R vi2 = pw2(jv[q].x) + pw2(jv[q].y) + pw2(jv[q].z);
+        const R ri = sqrt_r(pw2(jx[q].x) + pw2(jx[q].y) + pw2(jx[q].z));
+        R fa = (R(0.5) * vi2 + R(3.) * mu / ri) / c2;
+        for (int it = 0; it < 10; ++it) {
+            const V3<R> old = vi;
+            vi.x = jv[q].x / (R(1.) - fa); vi.y = jv[q].y / (R(1.) - fa); vi.z = jv[q].z / (R(1.) - fa);
+            vi2 = vi.x * vi.x + vi.y * vi.y + vi.z * vi.z;
+            fa = (R(0.5) * vi2 + R(3.) * mu / ri) / c2;
+            const R dvx = vi.x - old.x, dvy = vi.y - old.y, dvz = vi.z - old.z;
+            if (val((dvx * dvx + dvy * dvy + dvz * dvz) / vi2) < DBL_EPS_2) break;
+        }
+        const R ri3 = ri * ri * ri;
+        const R fb = (mu / ri - R(1.5) * vi2) * mu / ri3 / c2;
+        const R rdr = jx[q].x * jv[q].x + jx[q].y * jv[q].y + jx[q].z * jv[q].z;
+        const V3<R> vd{ja[q].x + fb * jx[q].x, ja[q].y + fb * jx[q].y, ja[q].z + fb * jx[q].z};
+        const R vdv = vi.x * vd.x + vi.y * vd.y + vi.z * vd.z;
+        const R fd = (vdv - R(3.) * mu / ri3 * rdr) / c2;
+        ja[q].x = fb * (R(1.) - fa) * jx[q].x - fa * ja[q].x - fd * vi.x;
+        ja[q].y = fb * (R(1.) - fa) * jx[q].y - fa * ja[q].y - fd * vi.y;
+        ja[q].z = fb * (R(1.) - fa) * jx[q].z - fa * ja[q].z - fd * vi.z;
+    }
+    // back to inertial accelerations, the Jacobi star acceleration being zero
+    eta = mtot;
+    sa = V3<R>{eta * R(0.), eta * R(0.), eta * R(0.)};
+    for (int q = np - 1; q >= 0; --q) {
+        Body<R> &P = S.p[idx[q]];
+        const R ei = R(1.) / eta;
+        sa.x = (sa.x - P.m * ja[q].x) * ei; sa.y = (sa.y - P.m * ja[q].y) * ei; sa.z = (sa.z - P.m * ja[q].z) * ei;
+        P.gr_acc = V3<R>{ja[q].x + sa.x, ja[q].y + sa.y, ja[q].z + sa.z};
+        eta = eta - P.m;
+        sa.x = sa.x * eta; sa.y = sa.y * eta; sa.z = sa.z * eta;
+    }
+    const R mi = R(1.) / eta;
+    H.gr_acc = V3<R>{sa.x * mi, sa.y * mi, sa.z * mi};
+}
+
+// Newhall et al. 1983, all pairs (general_relativity.rs:683-895).  Bodies
+// in the order host, then the others by index.
+template <class R> inline void newhall1983(System<R> &S) {
+    const int n = S.n;
+    int ord[MAXB];
+    ord[0] = S.host;
+    for (int i = 0, c = 1; i < n; ++i)
+        if (i != S.host) ord[c++] = i;
+    V3<R> an[MAXB];
+    gr_newtonian_accelerations(S, an);
+    V3<R> drs[MAXB][MAXB];
+    R rs[MAXB][MAXB];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            drs[i][j] = v3zero<R>();
+            rs[i][j] = R(0.);
+        }
+    auto gr_on = [&](int i) { return (S.p[ord[i]].roles & (POS_ROLE_GR_CENTRAL | POS_ROLE_GR_ORBITING)) != 0; };
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            if (j == i || !(gr_on(i) || gr_on(j))) continue;
+            const Body<R> &Pi = S.p[ord[i]], &Pj = S.p[ord[j]];
+            drs[i][j] = V3<R>{Pi.x.x - Pj.x.x, Pi.x.y - Pj.x.y, Pi.x.z - Pj.x.z};
+            rs[i][j] = sqrt_r(pw2(drs[i][j].x) + pw2(drs[i][j].y) + pw2(drs[i][j].z));
+        }
+    const R c2 = R(SPEED_OF_LIGHT_2);
+    V3<R> ac[MAXB];
+    for (int i = 0; i < n; ++i) {
+        const Body<R> &Pi = S.p[ord[i]];
+        V3<R> a = v3zero<R>();
+        for (int j = 0; j < n; ++j) {
+            if (j == i || !(gr_on(i) || gr_on(j))) continue;
+            const Body<R> &Pj = S.p[ord[j]];
+            const R dx = drs[i][j].x, dy = drs[i][j].y, dz = drs[i][j].z;
+            const R rij2 = pw2(rs[i][j]);
+            const R rij3 = rij2 * rs[i][j];
+            R a1 = R(0.);
+            for (int k = 0; k < n; ++k)
+                if (k != i) a1 = a1 + (R(4.) / c2) * R(G) * S.p[ord[k]].m / rs[i][k];
+            R a2 = R(0.);
+            for (int l = 0; l < n; ++l)
+                if (l != j) a2 = a2 + (R(1.) / c2) * R(G) * S.p[ord[l]].m / rs[l][j];
+            const R vi2 = pw2(Pi.v.x) + pw2(Pi.v.y) + pw2(Pi.v.z);
+            const R a3 = -vi2 / c2;
+            const R vj2 = pw2(Pj.v.x) + pw2(Pj.v.y) + pw2(Pj.v.z);
+            const R a4 = R(-2.) * vj2 / c2;
+            const R a5 = (R(4.) / c2) * (Pi.v.x * Pj.v.x + Pi.v.y * Pj.v.y + Pi.v.z * Pj.v.z);
+            const R a60 = dx * Pj.v.x + dy * Pj.v.y + dz * Pj.v.z;
+            const R a6 = (R(3.) / (R(2.) * c2)) * pw2(a60) / rij2;
+            const R f1 = a1 + a2 + a3 + a4 + a5 + a6;
+            const R gm = R(G) * Pj.m;
+            a.x = a.x + gm * dx * f1 / rij3;
+            a.y = a.y + gm * dy * f1 / rij3;
+            a.z = a.z + gm * dz * f1 / rij3;
+            const R dvx = Pi.v.x - Pj.v.x, dvy = Pi.v.y - Pj.v.y, dvz = Pi.v.z - Pj.v.z;
+            const R f2 = dx * (R(4.) * Pi.v.x - R(3.) * Pj.v.x) + dy * (R(4.) * Pi.v.y - R(3.) * Pj.v.y)
+                       + dz * (R(4.) * Pi.v.z - R(3.) * Pj.v.z);
+            a.x = a.x + gm * f2 * dvx / rij3 / c2;
+            a.y = a.y + gm * f2 * dvy / rij3 / c2;
+            a.z = a.z + gm * f2 * dvz / rij3 / c2;
+        }
+        ac[i] = a;
+    }
+    V3<R> anew[MAXB], aold[MAXB];
+    for (int i = 0; i < n; ++i) anew[i] = v3zero<R>();
+    const double dev_limit = 1.0e-30;
+    for (int k = 0; k < 10; ++k) {
+        for (int i = 0; i < n; ++i) aold[i] = anew[i];
+        for (int i = 0; i < n; ++i) {
+            V3<R> nc = v3zero<R>();
+            for (int j = 0; j < n; ++j) {
+                if (j == i || !(gr_on(i) || gr_on(j))) continue;
+                const Body<R> &Pj = S.p[ord[j]];
+                const R dx = drs[i][j].x, dy = drs[i][j].y, dz = drs[i][j].z;
+                const R rij = rs[i][j];
+                const R rij2 = pw2(rij);
+                const R rij3 = rij2 * rij;
+                const V3<R> s{an[j].x + aold[j].x, an[j].y + aold[j].y, an[j].z + aold[j].z};
+                const R proj = dx * s.x + dy * s.y + dz * s.z;
+                const R gm = R(G) * Pj.m;
+                nc.x = nc.x + (gm * dx / rij3) * proj / (R(2.) * c2) + (R(7.) / (R(2.) * c2)) * R(G) * Pj.m * s.x / rij;
+                nc.y = nc.y + (gm * dy / rij3) * proj / (R(2.) * c2) + (R(7.) / (R(2.) * c2)) * R(G) * Pj.m * s.y / rij;
+                nc.z = nc.z + (gm * dz / rij3) * proj / (R(2.) * c2) + (R(7.) / (R(2.) * c2)) * R(G) * Pj.m * s.z / rij;
+            }
+            anew[i] = V3<R>{ac[i].x + nc.x, ac[i].y + nc.y, ac[i].z + nc.z};
+        }
+        double maxdev = 0., dx = 0., dy = 0., dz = 0.;
+        for (int i = 0; i < n; ++i) {
+            if (!gr_on(i)) continue;
+            if (std::fabs(val(anew[i].x)) < dev_limit) dx = std::fabs(val(anew[i].x) - val(aold[i].x)) / val(anew[i].x);
+            if (std::fabs(val(anew[i].y)) < dev_limit) dy = std::fabs(val(anew[i].y) - val(aold[i].y)) / val(anew[i].y);
+            if (std::fabs(val(anew[i].z)) < dev_limit) dz = std::fabs(val(anew[i].z) - val(aold[i].z)) / val(anew[i].z);
+            if (dx > maxdev) maxdev = dx;
+            if (dy > maxdev) maxdev = dy;
+            if (dz > maxdev) maxdev = dz;
+        }
+        if (maxdev < dev_limit) break;
+    }
+    for (int i = 1; i < n; ++i) {
+        Body<R> &P = S.p[ord[i]];
+        if (P.roles & POS_ROLE_GR_ORBITING) P.gr_acc = anew[i];
+    }
+    S.p[S.host].gr_acc = anew[0];
+}
+
 template <class R> inline void wind_factor(System<R> &S) {
     for (int i = 0; i < S.n; ++i) {
         Body<R> &P = S.p[i];
@@ -779,6 +987,8 @@ inline void calculate_additional_effects(System<R> &S, const R &t, bool evolutio
         kidder_second_order(S);
         kidder_spin_orbit(S);
     }
+    if (accelerations && S.gr && S.gr_impl == POS_GR_ANDERSON1975) anderson1975(S);
+    if (accelerations && S.gr && S.gr_impl == POS_GR_NEWHALL1983) newhall1983(S);
     if (dangular_momentum_dt && (S.tides || S.rf || (S.gr && S.gr_impl == POS_GR_KIDDER1995) || S.wind)) {
         for (int i = 0; i < S.n; ++i) {
             Body<R> &P = S.p[i];
@@ -800,7 +1010,6 @@ inline void calculate_additional_effects(System<R> &S, const R &t, bool evolutio
 // ---------------------------------------------------------------------------
 // Newtonian gravity (universe.rs:198-303)
 // ---------------------------------------------------------------------------
-enum IgnoreTerms { IGNORE_NONE = 0, IGNORE_WHFAST_ONE = 1, IGNORE_WHFAST_TWO = 2 };
 
 template <class R> inline void gravity_calculate_acceleration(System<R> &S, int ignore) {
     const int mm = S.host;

@@ -15,8 +15,6 @@ from tests.oracle_runner import golden_names, load_golden
 
 UNSUPPORTED = {
     "test_disk-enabled_disk": "disk effect",
-    "test_general_relativity-anderson1975": "Kidder1995",
-    "test_general_relativity-newhall1983": "Kidder1995",
     "test_evolution-solar_like_bolmontmathis2016": "evolving dissipation",
     "test_evolution-solar_like_galletbolmont2017": "evolving dissipation",
 }
@@ -26,7 +24,7 @@ SUPPORTED = [n for n in golden_names() if n not in UNSUPPORTED]
 def test_fixture_inventory():
     # 26 WHFast fixture sets in the reference (IAS15/LeapFrog excluded)
     assert len(golden_names()) == 26
-    assert len(SUPPORTED) == 21
+    assert len(SUPPORTED) == 23
 
 
 @pytest.mark.parametrize("name", SUPPORTED)
