@@ -364,6 +364,7 @@ __device__ inline void adjoint_load(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     A.flat = eff & POS_EFFECT_ROTATIONAL_FLATTENING;
     A.gr = eff & POS_EFFECT_GENERAL_RELATIVITY;
     A.spin = eff & (POS_EFFECT_TIDES | POS_EFFECT_ROTATIONAL_FLATTENING | POS_EFFECT_GENERAL_RELATIVITY | POS_EFFECT_EVOLUTION);
+    // (the recording run admits no GR variant other than Kidder 1995)
     A.dt = E.dt[s];
     A.done = Ar.steps_done[s];
     const int b = A.body;

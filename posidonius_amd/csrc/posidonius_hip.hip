@@ -61,7 +61,10 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
     if (eff & POS_EFFECT_GENERAL_RELATIVITY) f |= F_GR;
     if (eff & POS_EFFECT_WIND) f |= F_WIND;
     if (eff & POS_EFFECT_EVOLUTION) f |= F_EVOLUTION;
-    if (eff & (POS_EFFECT_TIDES | POS_EFFECT_ROTATIONAL_FLATTENING | POS_EFFECT_EVOLUTION | POS_EFFECT_GENERAL_RELATIVITY))
+    // spins are integrated with tides, flattening, evolution or Kidder GR
+    // (whfast.rs:259-262); the other GR variants only add accelerations
+    if ((eff & (POS_EFFECT_TIDES | POS_EFFECT_ROTATIONAL_FLATTENING | POS_EFFECT_EVOLUTION)) ||
+        ((eff & POS_EFFECT_GENERAL_RELATIVITY) && eff_gr_variant(eff) == GRV_KIDDER))
         f |= F_SPIN;
     if (roles & POS_ROLE_WIND) f |= F_WIND_ROLE;
     if (B.evo == POS_EVO_LECONTE_CHABRIER2013 && (roles & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_TIDES_ORBITING)))
@@ -141,19 +144,21 @@ __device__ inline void store_lane(const DevEnsemble &E, const WaveLds &lds, cons
 
 constexpr uint32_t FROZEN = POS_STATUS_COMPLETED | POS_STATUS_FATAL;
 
-// K steps of every system of every wave; the grid may be smaller than the
-// number of waves, in which case each workgroup walks the table with a
-// grid stride (persistent launch).
+// K steps of every system of waves [wave_begin, wave_end), all of GR variant
+// GRV (general relativity off or Kidder 1995 for GRV_KIDDER); the grid may be
+// smaller than the number of waves, in which case each workgroup walks the
+// table with a grid stride (persistent launch).
+template <int GRV>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POS_WAVES_PER_EU)))
-void whfast_steps_kernel(DevEnsemble E, int n_steps) {
+void whfast_steps_kernel(DevEnsemble E, int n_steps, int wave_begin, int wave_end) {
     __shared__ WaveLds lds;
-    for (int wave = blockIdx.x; wave < E.n_waves; wave += gridDim.x) {
+    for (int wave = wave_begin + blockIdx.x; wave < wave_end; wave += gridDim.x) {
         Lane B;
         load_lane(E, lds, B, wave);
         const TapeRef none{nullptr, 0};
         for (int step = 0; step < n_steps; ++step) {
             if (B.status & FROZEN) break;   // uniform within the group
-            iterate<false>(E, lds, B, none, 0);
+            iterate<false, GRV>(E, lds, B, none, 0);
         }
         store_lane(E, lds, B);
         __syncthreads();
@@ -170,7 +175,7 @@ void whfast_record_kernel(DevEnsemble E, int n_steps, TapeRef T) {
         load_lane(E, lds, B, wave);
         for (int step = 0; step < n_steps; ++step) {
             if (B.status & FROZEN) break;   // uniform within the group
-            iterate<true>(E, lds, B, T, step);
+            iterate<true, GRV_KIDDER>(E, lds, B, T, step);   // the adjoint excludes the other GR variants
         }
         store_lane(E, lds, B);
         __syncthreads();
@@ -225,14 +230,16 @@ __global__ __launch_bounds__(64) void gravity_kernel(DevEnsemble E, double *out_
     }
 }
 
-__global__ __launch_bounds__(64) void effects_kernel(DevEnsemble E, double *out_acc, double *out_dLdt) {
+template <int GRV>
+__global__ __launch_bounds__(64) void effects_kernel(DevEnsemble E, double *out_acc, double *out_dLdt, int wave_begin,
+                                                     int wave_end) {
     __shared__ WaveLds lds;
-    const int wave = blockIdx.x;
-    if (wave >= E.n_waves) return;
+    const int wave = wave_begin + blockIdx.x;
+    if (wave >= wave_end) return;
     Lane B;
     load_lane(E, lds, B, wave);
     double a[3], dL[3];
-    additional_effects(E, lds, B, false, a, dL);
+    additional_effects<GRV>(E, lds, B, false, a, dL);
     const int nb = E.n_bodies;
     if (B.valid()) {
 #pragma unroll
@@ -281,6 +288,8 @@ struct pos_ens {
     std::vector<int> n_per_sys_dev;
     // launch tables per mapping
     std::vector<int> wave_first, wave_count, wave_group;
+    int variant_waves[3][2] = {{0, 0}, {0, 0}, {0, 0}};   // [GR variant] -> wave range [begin, end)
+    std::vector<int> variant_dev;                          // GR variant of each device system
     int *d_wave_first = nullptr, *d_wave_count = nullptr, *d_wave_group = nullptr;   // capacity n_systems
     double *d_op_a = nullptr, *d_op_l = nullptr;   // operator outputs, [3][n_bodies]
     uint32_t *d_op_st = nullptr;                    // operator status, [n_systems]
@@ -374,12 +383,12 @@ int build_launch(pos_ens *h, const pos_options &o) {
             int G = group_lanes(o.group_size, n);
             if (G > 64) G = 64;
             const int per_wave = 64 / G;
-            // systems with the same group size share waves
+            // systems with the same group size and GR variant share waves
             int e = s;
             while (e < ns) {
                 const int ne = h->n_per_sys_dev[e];
                 const int Ge = group_lanes(o.group_size, ne);
-                if (std::min(Ge, 64) != G) break;
+                if (std::min(Ge, 64) != G || h->variant_dev[e] != h->variant_dev[s]) break;
                 ++e;
             }
             for (int f = s; f < e; f += per_wave) {
@@ -397,6 +406,16 @@ int build_launch(pos_ens *h, const pos_options &o) {
     std::reverse(h->wave_count.begin(), h->wave_count.end());
     std::reverse(h->wave_group.begin(), h->wave_group.end());
     const int nw = (int)h->wave_first.size();
+    // device order groups the variants, so each one covers a contiguous range
+    for (int v = 0; v < 3; ++v) {
+        h->variant_waves[v][0] = nw;
+        h->variant_waves[v][1] = 0;
+    }
+    for (int w = 0; w < nw; ++w) {
+        const int v = h->variant_dev[h->wave_first[w]];
+        h->variant_waves[v][0] = std::min(h->variant_waves[v][0], w);
+        h->variant_waves[v][1] = std::max(h->variant_waves[v][1], w + 1);
+    }
     int rc;
     if (!h->d_wave_first) {
         if ((rc = dalloc(h, &h->d_wave_first, ns)) || (rc = dalloc(h, &h->d_wave_count, ns)) ||
@@ -419,6 +438,14 @@ int build_launch(pos_ens *h, const pos_options &o) {
     return POS_OK;
 }
 
+// GR variant a system's steps are compiled for (GRV_KIDDER also when GR is off)
+int gr_variant(uint32_t effects, int implementation) {
+    if (!(effects & POS_EFFECT_GENERAL_RELATIVITY)) return GRV_KIDDER;
+    if (implementation == POS_GR_ANDERSON1975) return GRV_ANDERSON;
+    if (implementation == POS_GR_NEWHALL1983) return GRV_NEWHALL;
+    return GRV_KIDDER;
+}
+
 int validate(const pos_ensemble_desc *d) {
     if (!d) return fail(POS_E_INVALID, "desc is NULL");
     if (d->n_systems <= 0) return fail(POS_E_INVALID, "n_systems must be positive");
@@ -437,8 +464,10 @@ int validate(const pos_ensemble_desc *d) {
         const uint32_t eff = d->effects[s];
         if (eff & POS_EFFECT_DISK) return fail(POS_E_UNSUPPORTED, where + "disk effect is not supported");
         if (d->coordinates[s] < 0 || d->coordinates[s] > 2) return fail(POS_E_INVALID, where + "bad coordinates");
-        if ((eff & POS_EFFECT_GENERAL_RELATIVITY) && d->gr_implementation[s] != POS_GR_KIDDER1995)
-            return fail(POS_E_UNSUPPORTED, where + "only Kidder1995 general relativity is supported");
+        if ((eff & POS_EFFECT_GENERAL_RELATIVITY) &&
+            (d->gr_implementation[s] < POS_GR_KIDDER1995 || d->gr_implementation[s] > POS_GR_DISABLED))
+            return fail(POS_E_INVALID, where + "bad general relativity implementation");
+        const bool gr_on = (eff & POS_EFFECT_GENERAL_RELATIVITY) && d->gr_implementation[s] != POS_GR_DISABLED;
         if (!(d->time_step[s] > 0.)) return fail(POS_E_INVALID, where + "time_step must be positive");
         int mm = -1;
         double mmax = 0.;
@@ -450,7 +479,7 @@ int validate(const pos_ensemble_desc *d) {
             return fail(POS_E_UNSUPPORTED, where + "the tidal host must be the most massive body");
         if ((eff & POS_EFFECT_ROTATIONAL_FLATTENING) && !(hr & POS_ROLE_FLAT_CENTRAL))
             return fail(POS_E_UNSUPPORTED, where + "the flattening host must be the most massive body");
-        if ((eff & POS_EFFECT_GENERAL_RELATIVITY) && !(hr & POS_ROLE_GR_CENTRAL))
+        if (gr_on && !(hr & POS_ROLE_GR_CENTRAL))
             return fail(POS_E_UNSUPPORTED, where + "the GR host must be the most massive body");
         for (int i = 0; i < n; ++i) {
             if (i == mm) continue;
@@ -461,7 +490,7 @@ int validate(const pos_ensemble_desc *d) {
                 return fail(POS_E_UNSUPPORTED, where + "with tides on every other body must be a tidal orbiting body");
             if ((eff & POS_EFFECT_ROTATIONAL_FLATTENING) && !(r & POS_ROLE_FLAT_ORBITING))
                 return fail(POS_E_UNSUPPORTED, where + "with flattening on every other body must be an orbiting body");
-            if ((eff & POS_EFFECT_GENERAL_RELATIVITY) && !(r & POS_ROLE_GR_ORBITING))
+            if (gr_on && !(r & POS_ROLE_GR_ORBITING))
                 return fail(POS_E_UNSUPPORTED, where + "with GR on every other body must be an orbiting body");
         }
         const double last = d->last_historic_snapshot_time ? d->last_historic_snapshot_time[s] : -1.;
@@ -529,9 +558,20 @@ int adjoint_copy_state(pos_ens *h, char *slot, bool save) {
 
 constexpr int GRAPH_LAUNCHES = 4;   // kernel launches per captured graph
 
+// One launch per GR variant present; the persistent grid size caps each.
 int launch_steps(pos_ens *h, int n_steps) {
-    hipLaunchKernelGGL(whfast_steps_kernel, dim3(h->grid), dim3(64), 0, h->stream, h->E, n_steps);
-    HIP_TRY(hipGetLastError());
+    for (int v = 0; v < 3; ++v) {
+        const int w0 = h->variant_waves[v][0], w1 = h->variant_waves[v][1];
+        if (w1 <= w0) continue;
+        const int grid = std::min(h->grid, w1 - w0);
+        if (v == GRV_KIDDER)
+            hipLaunchKernelGGL(whfast_steps_kernel<GRV_KIDDER>, dim3(grid), dim3(64), 0, h->stream, h->E, n_steps, w0, w1);
+        else if (v == GRV_ANDERSON)
+            hipLaunchKernelGGL(whfast_steps_kernel<GRV_ANDERSON>, dim3(grid), dim3(64), 0, h->stream, h->E, n_steps, w0, w1);
+        else
+            hipLaunchKernelGGL(whfast_steps_kernel<GRV_NEWHALL>, dim3(grid), dim3(64), 0, h->stream, h->E, n_steps, w0, w1);
+        HIP_TRY(hipGetLastError());
+    }
     return POS_OK;
 }
 
@@ -567,12 +607,17 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     h->n_bodies = nb;
     h->body_offset_in.assign(d->body_offset, d->body_offset + ns + 1);
 
-    // device order: by body count, then input order
+    // device order: by GR variant, then body count, then input order
+    std::vector<int> variant_in(ns);
+    for (int s = 0; s < ns; ++s) variant_in[s] = gr_variant(d->effects[s], d->gr_implementation[s]);
     h->sys_perm.resize(ns);
     std::iota(h->sys_perm.begin(), h->sys_perm.end(), 0);
     std::stable_sort(h->sys_perm.begin(), h->sys_perm.end(), [&](int a, int b) {
+        if (variant_in[a] != variant_in[b]) return variant_in[a] < variant_in[b];
         return (d->body_offset[a + 1] - d->body_offset[a]) < (d->body_offset[b + 1] - d->body_offset[b]);
     });
+    h->variant_dev.resize(ns);
+    for (int ds = 0; ds < ns; ++ds) h->variant_dev[ds] = variant_in[h->sys_perm[ds]];
     h->body_offset_dev.assign(ns + 1, 0);
     h->n_per_sys_dev.resize(ns);
     h->body_perm.resize(nb);
@@ -599,7 +644,8 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
         host_idx[ds] = mm;
         coord[ds] = d->coordinates[s];
         uint32_t e = d->effects[s];
-        eff[ds] = e;
+        if (d->gr_implementation[s] == POS_GR_DISABLED) e &= ~POS_EFFECT_GENERAL_RELATIVITY;
+        eff[ds] = e | ((uint32_t)h->variant_dev[ds] << EFF_GRV_SHIFT);
         dt[ds] = d->time_step[s];
         tl[ds] = d->time_limit[s];
         t[ds] = d->current_time ? d->current_time[s] : 0.;
@@ -745,9 +791,10 @@ int32_t pos_step(pos_ens *h, int64_t n_steps) {
         // GRAPH_LAUNCHES chunk launches captured once per launch table, replayed
         if (!h->graph_exec) {
             HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < GRAPH_LAUNCHES; ++i)
-                hipLaunchKernelGGL(whfast_steps_kernel, dim3(h->grid), dim3(64), 0, h->stream, h->E, K);
+            int crc = POS_OK;
+            for (int i = 0; i < GRAPH_LAUNCHES && crc == POS_OK; ++i) crc = launch_steps(h, K);
             HIP_TRY(hipStreamEndCapture(h->stream, &h->graph));
+            if (crc) return crc;
             HIP_TRY(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
             h->graph_launches = GRAPH_LAUNCHES;
         }
@@ -997,8 +1044,17 @@ int32_t pos_additional_effects(pos_ens *h, double *out_acc, double *out_dLdt) {
                        (rc = dalloc(h, &h->d_op_st, h->n_systems))))
         return rc;
     double *d_a = h->d_op_a, *d_l = h->d_op_l;
-    hipLaunchKernelGGL(effects_kernel, dim3(h->E.n_waves), dim3(64), 0, h->stream, h->E, d_a, d_l);
-    HIP_TRY(hipGetLastError());
+    for (int v = 0; v < 3; ++v) {
+        const int w0 = h->variant_waves[v][0], w1 = h->variant_waves[v][1];
+        if (w1 <= w0) continue;
+        if (v == GRV_KIDDER)
+            hipLaunchKernelGGL(effects_kernel<GRV_KIDDER>, dim3(w1 - w0), dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1);
+        else if (v == GRV_ANDERSON)
+            hipLaunchKernelGGL(effects_kernel<GRV_ANDERSON>, dim3(w1 - w0), dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1);
+        else
+            hipLaunchKernelGGL(effects_kernel<GRV_NEWHALL>, dim3(w1 - w0), dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1);
+        HIP_TRY(hipGetLastError());
+    }
     std::vector<double> a, l;
     if ((rc = download(h, a, d_a, 3 * (size_t)nb)) || (rc = download(h, l, d_l, 3 * (size_t)nb))) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1188,6 +1244,8 @@ int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
             return fail(POS_E_UNSUPPORTED, where + "the adjoint supports democratic-heliocentric coordinates only");
         if (h->eff_dev[ds] & (POS_EFFECT_WIND | POS_EFFECT_DISK | POS_EFFECT_EVOLUTION))
             return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support wind, disk or evolution");
+        if (h->variant_dev[ds] != GRV_KIDDER)
+            return fail(POS_E_UNSUPPORTED, where + "the adjoint supports Kidder1995 general relativity only");
         for (int b = h->body_offset_dev[ds]; b < h->body_offset_dev[ds + 1]; ++b)
             if (h->evo_dev[b] != POS_EVO_NON_EVOLVING)
                 return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support evolving bodies");

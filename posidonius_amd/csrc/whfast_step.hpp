@@ -270,10 +270,274 @@ __device__ __forceinline__ void host_reaction_sums(WaveLds &lds, const Lane &B) 
     }
 }
 
+// ===========================================================================
+// General-relativity variants other than Kidder 1995.  They run after the
+// host reactions of the other effects (their accelerations are added last,
+// universe.rs:540-566) and use the exchange rows freely.
+// ===========================================================================
+enum GrVariant { GRV_KIDDER = 0, GRV_ANDERSON = 1, GRV_NEWHALL = 2 };
+// The device copy of a system's effect mask carries its variant in these bits.
+constexpr uint32_t EFF_GRV_SHIFT = 16;
+__host__ __device__ __forceinline__ int eff_gr_variant(uint32_t eff) { return (int)((eff >> EFF_GRV_SHIFT) & 3u); }
+
+// Chained order of general_relativity.rs (host, then the other bodies by
+// index): position of body k, and body at position c.
+__device__ __forceinline__ int chained_pos(const Lane &B, int k) { return k == B.host ? 0 : (k < B.host ? k + 1 : k); }
+__device__ __forceinline__ int chained_body(const Lane &B, int c) { return c == 0 ? B.host : (c - 1 < B.host ? c - 1 : c); }
+
+// general_relativity.rs:641-681: the accelerations of the last gravity
+// evaluation plus the host pair terms the splitting ignored; the separation
+// mixes the host's inertial position with the body's heliocentric position,
+// as the reference does.  Rows 0-2 must hold every body's position.  Leaves
+// this body's acceleration in an[]; uses rows 3-5.
+__device__ inline void gr_newtonian(WaveLds &lds, Lane &B, double an[3]) {
+    const int lane = B.lane, hl = B.hl();
+#pragma unroll
+    for (int q = 0; q < 3; ++q) an[q] = lds.ag[q][lane];
+    // the ignored pairs: the first body after the host in chained order
+    // (Jacobi) or every body (heliocentric splittings)
+    const int np = B.coord == POS_COORD_JACOBI ? 1 : B.n - 1;
+    const int c = chained_pos(B, B.k);
+    double host_term[3] = {0., 0., 0.};
+    if (B.orbiting() && c <= np) {   // every body of a GR system is a GR body
+        const double hx[3] = {B.x[0] - lds.xb[0][hl], B.x[1] - lds.xb[1][hl], B.x[2] - lds.xb[2][hl]};
+        const double dx = lds.xb[0][hl] - hx[0], dy = lds.xb[1][hl] - hx[1], dz = lds.xb[2][hl] - hx[2];
+        const double r2 = p2(dx) + p2(dy) + p2(dz);
+        const double r = sqrt(r2);
+        const double prefac = G / (r2 * r);
+        const double pms = prefac * lds.m[hl], pmp = prefac * B.m;
+        an[0] = an[0] + pms * dx; an[1] = an[1] + pms * dy; an[2] = an[2] + pms * dz;
+        host_term[0] = pmp * dx; host_term[1] = pmp * dy; host_term[2] = pmp * dz;
+    }
+    if (B.valid()) { lds.xb[3][lane] = host_term[0]; lds.xb[4][lane] = host_term[1]; lds.xb[5][lane] = host_term[2]; }
+    wsync();
+    if (B.valid() && B.is_host()) {
+        for (int cc = 1; cc <= np && cc < B.n; ++cc) {
+            const int l = B.gbase + chained_body(B, cc);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) an[q] = an[q] - lds.xb[3 + q][l];
+        }
+    }
+    wsync();
+}
+
+// Anderson et al. 1975 through Jacobi coordinates (general_relativity.rs:461-639)
+__device__ inline void gr_anderson1975(WaveLds &lds, Lane &B, double g[3]) {
+    const int lane = B.lane, hl = B.hl();
+    if (B.valid()) { lds.xb[0][lane] = B.x[0]; lds.xb[1][lane] = B.x[1]; lds.xb[2][lane] = B.x[2]; }
+    wsync();
+    double an[3];
+    gr_newtonian(lds, B, an);
+    if (B.valid()) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) { lds.xb[3 + q][lane] = B.v[q]; lds.xb[6 + q][lane] = an[q]; }
+    }
+    wsync();
+    // Jacobi prefix recurrence up to this body, recomputed by every lane
+    double eta = lds.m[hl];
+    double s[3], sv[3], sa[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) { s[q] = eta * lds.xb[q][hl]; sv[q] = eta * lds.xb[3 + q][hl]; sa[q] = eta * lds.xb[6 + q][hl]; }
+    double jx[3] = {0., 0., 0.}, jv[3] = {0., 0., 0.}, ja[3] = {0., 0., 0.};
+    for (int j = 0; j < B.n; ++j) {
+        if (j == B.host) continue;
+        const int l = B.gbase + j;
+        const double mj = lds.m[l];
+        const double ei = 1. / eta;
+        eta = eta + mj;
+        const double pme = eta * ei;
+        double x[3], v[3], a[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            x[q] = lds.xb[q][l] - s[q] * ei;
+            v[q] = lds.xb[3 + q][l] - sv[q] * ei;
+            a[q] = lds.xb[6 + q][l] - sa[q] * ei;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            s[q] = s[q] * pme + mj * x[q];
+            sv[q] = sv[q] * pme + mj * v[q];
+            sa[q] = sa[q] * pme + mj * a[q];
+        }
+        if (j == B.k) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) { jx[q] = x[q]; jv[q] = v[q]; ja[q] = a[q]; }
+        }
+    }
+    const double mtot = eta;
+    wsync();
+    if (B.orbiting()) {
+        const double mu = lds.mg[hl];
+        double vi[3] = {jv[0], jv[1], jv[2]};
+        double vi2 = p2(jv[0]) + p2(jv[1]) + p2(jv[2]);
+        const double ri = sqrt(p2(jx[0]) + p2(jx[1]) + p2(jx[2]));
+        double fa = (0.5 * vi2 + 3. * mu / ri) / SPEED_OF_LIGHT_2;
+        for (int it = 0; it < 10; ++it) {
+            const double o0 = vi[0], o1 = vi[1], o2 = vi[2];
+            vi[0] = jv[0] / (1. - fa); vi[1] = jv[1] / (1. - fa); vi[2] = jv[2] / (1. - fa);
+            vi2 = vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2];
+            fa = (0.5 * vi2 + 3. * mu / ri) / SPEED_OF_LIGHT_2;
+            const double dvx = vi[0] - o0, dvy = vi[1] - o1, dvz = vi[2] - o2;
+            if ((dvx * dvx + dvy * dvy + dvz * dvz) / vi2 < DBL_EPS_2) break;
+        }
+        const double ri3 = ri * ri * ri;
+        const double fb = (mu / ri - 1.5 * vi2) * mu / ri3 / SPEED_OF_LIGHT_2;
+        const double rdr = jx[0] * jv[0] + jx[1] * jv[1] + jx[2] * jv[2];
+        const double vd0 = ja[0] + fb * jx[0], vd1 = ja[1] + fb * jx[1], vd2 = ja[2] + fb * jx[2];
+        const double vdv = vi[0] * vd0 + vi[1] * vd1 + vi[2] * vd2;
+        const double fd = (vdv - 3. * mu / ri3 * rdr) / SPEED_OF_LIGHT_2;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) ja[q] = fb * (1. - fa) * jx[q] - fa * ja[q] - fd * vi[q];
+        lds.xb[9][lane] = ja[0]; lds.xb[10][lane] = ja[1]; lds.xb[11][lane] = ja[2];
+    }
+    wsync();
+    // reverse recurrence back to inertial accelerations (Jacobi star acceleration zero)
+    eta = mtot;
+    double t[3] = {eta * 0., eta * 0., eta * 0.};
+    g[0] = g[1] = g[2] = 0.;
+    for (int j = B.n - 1; j >= 0; --j) {
+        if (j == B.host) continue;
+        const int l = B.gbase + j;
+        const double mj = lds.m[l];
+        const double ei = 1. / eta;
+        double a[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            t[q] = (t[q] - mj * lds.xb[9 + q][l]) * ei;
+            a[q] = lds.xb[9 + q][l] + t[q];
+        }
+        if (j == B.k) { g[0] = a[0]; g[1] = a[1]; g[2] = a[2]; }
+        eta = eta - mj;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) t[q] = t[q] * eta;
+    }
+    if (B.valid() && B.is_host()) {
+        const double mi = 1. / eta;
+        g[0] = t[0] * mi; g[1] = t[1] * mi; g[2] = t[2] * mi;
+    }
+    wsync();
+}
+
+// Newhall et al. 1983, every pair (general_relativity.rs:683-895)
+__device__ inline void gr_newhall1983(WaveLds &lds, Lane &B, double g[3]) {
+    const int lane = B.lane;
+    const int n = B.n;
+    if (B.valid()) { lds.xb[0][lane] = B.x[0]; lds.xb[1][lane] = B.x[1]; lds.xb[2][lane] = B.x[2]; }
+    wsync();
+    double an[3];
+    gr_newtonian(lds, B, an);
+    if (B.valid()) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) { lds.xb[3 + q][lane] = B.v[q]; lds.xb[6 + q][lane] = an[q]; }
+    }
+    const double c2 = SPEED_OF_LIGHT_2;
+    // distance to the body at chained position c (0 for itself)
+    auto dist = [&](int c) {
+        const int l = B.gbase + chained_body(B, c);
+        const double dx = B.x[0] - lds.xb[0][l], dy = B.x[1] - lds.xb[1][l], dz = B.x[2] - lds.xb[2][l];
+        return sqrt(p2(dx) + p2(dy) + p2(dz));
+    };
+    const int me = chained_pos(B, B.k);
+    wsync();
+    // a2 depends only on the second body j: sum_l G m_l / (c^2 r_lj), which
+    // body j evaluates from its own distances (|x_l - x_j| = |x_j - x_l|)
+    double a1 = 0., a2own = 0.;
+    if (B.valid()) {
+        for (int c = 0; c < n; ++c) {
+            if (c == me) continue;
+            const double mk = lds.m[B.gbase + chained_body(B, c)];
+            const double r = dist(c);
+            a1 = a1 + (4. / c2) * G * mk / r;
+            a2own = a2own + (1. / c2) * G * mk / r;
+        }
+        lds.xb[9][lane] = a2own;
+    }
+    wsync();
+    double ac[3] = {0., 0., 0.};
+    if (B.valid()) {
+        const double vi2 = p2(B.v[0]) + p2(B.v[1]) + p2(B.v[2]);
+        for (int c = 0; c < n; ++c) {
+            if (c == me) continue;
+            const int l = B.gbase + chained_body(B, c);
+            const double dx = B.x[0] - lds.xb[0][l], dy = B.x[1] - lds.xb[1][l], dz = B.x[2] - lds.xb[2][l];
+            const double rij = sqrt(p2(dx) + p2(dy) + p2(dz));
+            const double rij2 = p2(rij);
+            const double rij3 = rij2 * rij;
+            const double vj0 = lds.xb[3][l], vj1 = lds.xb[4][l], vj2c = lds.xb[5][l];
+            const double a2 = lds.xb[9][l];
+            const double a3 = -vi2 / c2;
+            const double vj2 = p2(vj0) + p2(vj1) + p2(vj2c);
+            const double a4 = -2. * vj2 / c2;
+            const double a5 = (4. / c2) * (B.v[0] * vj0 + B.v[1] * vj1 + B.v[2] * vj2c);
+            const double a60 = dx * vj0 + dy * vj1 + dz * vj2c;
+            const double a6 = (3. / (2. * c2)) * p2(a60) / rij2;
+            const double f1 = a1 + a2 + a3 + a4 + a5 + a6;
+            const double gm = G * lds.m[l];
+            ac[0] = ac[0] + gm * dx * f1 / rij3;
+            ac[1] = ac[1] + gm * dy * f1 / rij3;
+            ac[2] = ac[2] + gm * dz * f1 / rij3;
+            const double dvx = B.v[0] - vj0, dvy = B.v[1] - vj1, dvz = B.v[2] - vj2c;
+            const double f2 = dx * (4. * B.v[0] - 3. * vj0) + dy * (4. * B.v[1] - 3. * vj1) + dz * (4. * B.v[2] - 3. * vj2c);
+            ac[0] = ac[0] + gm * f2 * dvx / rij3 / c2;
+            ac[1] = ac[1] + gm * f2 * dvy / rij3 / c2;
+            ac[2] = ac[2] + gm * f2 * dvz / rij3 / c2;
+        }
+    }
+    // fixed point: rows 10-12 hold the previous iterate of every body, rows
+    // 13-15 the new one.  A group stops iterating when its own test passes
+    // (every lane of the group evaluates it identically); the wavefront
+    // leaves the loop once every group has.
+    double anew[3] = {0., 0., 0.};
+    bool done = !B.valid();
+    for (int k = 0; k < 10; ++k) {
+        if (B.valid()) { lds.xb[10][lane] = anew[0]; lds.xb[11][lane] = anew[1]; lds.xb[12][lane] = anew[2]; }
+        wsync();
+        double nc[3] = {0., 0., 0.};
+        if (!done) {
+            for (int c = 0; c < n; ++c) {
+                if (c == me) continue;
+                const int l = B.gbase + chained_body(B, c);
+                const double dx = B.x[0] - lds.xb[0][l], dy = B.x[1] - lds.xb[1][l], dz = B.x[2] - lds.xb[2][l];
+                const double rij = sqrt(p2(dx) + p2(dy) + p2(dz));
+                const double rij2 = p2(rij);
+                const double rij3 = rij2 * rij;
+                const double s0 = lds.xb[6][l] + lds.xb[10][l], s1 = lds.xb[7][l] + lds.xb[11][l], s2 = lds.xb[8][l] + lds.xb[12][l];
+                const double proj = dx * s0 + dy * s1 + dz * s2;
+                const double gm = G * lds.m[l];
+                nc[0] = nc[0] + (gm * dx / rij3) * proj / (2. * c2) + (7. / (2. * c2)) * G * lds.m[l] * s0 / rij;
+                nc[1] = nc[1] + (gm * dy / rij3) * proj / (2. * c2) + (7. / (2. * c2)) * G * lds.m[l] * s1 / rij;
+                nc[2] = nc[2] + (gm * dz / rij3) * proj / (2. * c2) + (7. / (2. * c2)) * G * lds.m[l] * s2 / rij;
+            }
+        }
+        if (!done) { anew[0] = ac[0] + nc[0]; anew[1] = ac[1] + nc[1]; anew[2] = ac[2] + nc[2]; }
+        if (B.valid()) { lds.xb[13][lane] = anew[0]; lds.xb[14][lane] = anew[1]; lds.xb[15][lane] = anew[2]; }
+        wsync();
+        // convergence test of the reference, in its order (the deviations
+        // persist across bodies and are only updated for tiny components)
+        const double dev_limit = 1.0e-30;
+        double maxdev = 0., ddx = 0., ddy = 0., ddz = 0.;
+        for (int c = 0; !done && c < n; ++c) {
+            const int l = B.gbase + chained_body(B, c);
+            const double nx = lds.xb[13][l], ny = lds.xb[14][l], nz = lds.xb[15][l];
+            if (fabs(nx) < dev_limit) ddx = fabs(nx - lds.xb[10][l]) / nx;
+            if (fabs(ny) < dev_limit) ddy = fabs(ny - lds.xb[11][l]) / ny;
+            if (fabs(nz) < dev_limit) ddz = fabs(nz - lds.xb[12][l]) / nz;
+            if (ddx > maxdev) maxdev = ddx;
+            if (ddy > maxdev) maxdev = ddy;
+            if (ddz > maxdev) maxdev = ddz;
+        }
+        if (maxdev < dev_limit) done = true;
+        wsync();
+        if (__all(done)) break;
+    }
+    g[0] = anew[0]; g[1] = anew[1]; g[2] = anew[2];
+}
+
 // Accelerations and spin derivatives of every body of the group at the current
 // (averaged) state.  Outputs are accumulated in the reference's order:
 // a_add = ((0 + tides) + flattening) + GR and
 // dL/dt = ((tides + flattening) + GR) + wind (universe.rs:540-614).
+template <int GRV>
 __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, Lane &B, bool evolution,
                                           double a_add[3], double dLdt[3]) {
     const int lane = B.lane;
@@ -405,7 +669,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
 #pragma unroll
             for (int q = 0; q < 3; ++q) dLdt[q] = tdL[q] + rdL[q];
         }
-        if (B.gr()) {
+        if (GRV == GRV_KIDDER && B.gr()) {
             // Kidder 1995 (general_relativity.rs:177-456)
             const Rcp Rhm{hm, lds.inv_m[hl]};
             const Rcp Rm{m, B.inv_m};
@@ -525,7 +789,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
 #pragma unroll
             for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + 0.;
         }
-        if (B.gr()) {      // general_relativity.rs:236-238, 295-297, 453-455, 312-315
+        if (GRV == GRV_KIDDER && B.gr()) {      // general_relativity.rs:236-238, 295-297, 453-455, 312-315
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 double g = -1.0 * row(C_G1 + q);
@@ -540,6 +804,16 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         }
     }
     wsync();
+    // ---- the other GR implementations: accelerations only, added last ----
+    if (GRV != GRV_KIDDER) {   // uniform: such a launch holds only systems of this variant
+        double g[3];
+        if (GRV == GRV_ANDERSON) gr_anderson1975(lds, B, g);
+        else gr_newhall1983(lds, B, g);
+        if (B.valid() && B.gr()) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + g[q];
+        }
+    }
     // ---- wind (wind.rs:61-91), own body, added last ----
     double w[3] = {0., 0., 0.};
     if (B.valid() && (B.flags & F_WIND) && B.spin_on() && (B.flags & F_WIND_ROLE)) {
@@ -556,6 +830,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
     for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + w[q];
 }
 
+template <int GRV>
 __device__ inline void integrate_velocity_dependent_forces(const DevEnsemble &E, WaveLds &lds, Lane &B, bool evolution) {
     const double h = 0.5 * B.dt;
     const bool spin_on = B.spin_on();
@@ -565,7 +840,7 @@ __device__ inline void integrate_velocity_dependent_forces(const DevEnsemble &E,
     bool converged = false;
     for (int it = 0; it < MIDPOINT_MAX_ITER; ++it) {
         double a_add[3], dLdt[3];
-        additional_effects(E, lds, B, evolution && it == 0, a_add, dLdt);
+        additional_effects<GRV>(E, lds, B, evolution && it == 0, a_add, dLdt);
         double pdv = 0., pdL = 0.;   // |change of the final state|^2 of this body
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -1219,7 +1494,7 @@ __device__ __forceinline__ void tape_put3(const TapeRef &T, int row, int field, 
 // WHFast::iterate (whfast.rs:235-305).  With REC, the step also writes its
 // tape row `row` (and the end state into row + 1).
 // ===========================================================================
-template <bool REC>
+template <bool REC, int GRV>
 __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, const TapeRef &T, int row) {
     {
         const bool first = B.last_hist < 0.;
@@ -1238,7 +1513,7 @@ __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, cons
         tape_put3(T, row, TP_V, B.body, B.v);
         tape_put3(T, row, TP_L, B.body, B.L);
     }
-    integrate_velocity_dependent_forces(E, lds, B, true);
+    integrate_velocity_dependent_forces<GRV>(E, lds, B, true);
     if (REC && B.valid()) {
         tape_put3(T, row, TP_V1, B.body, B.v);
         tape_put3(T, row, TP_L1, B.body, B.L);
@@ -1254,7 +1529,7 @@ __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, cons
     kepler_steps(lds, B, half_dt);
     if (REC && B.valid()) *T.at(row, TP_XK2, B.body) = B.orbiting() ? B.kx : 0.;
     alternative_to_inertial(lds, B);
-    integrate_velocity_dependent_forces(E, lds, B, false);
+    integrate_velocity_dependent_forces<GRV>(E, lds, B, false);
     if (REC && B.valid()) {
         tape_put3(T, row + 1, TP_X, B.body, B.x);
         tape_put3(T, row + 1, TP_V, B.body, B.v);

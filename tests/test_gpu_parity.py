@@ -16,8 +16,7 @@ from tests.oracle_runner import golden_names, load_golden
 
 pytestmark = pytest.mark.gpu
 
-UNSUPPORTED = {"test_disk-enabled_disk", "test_general_relativity-anderson1975",
-               "test_general_relativity-newhall1983", "test_evolution-solar_like_bolmontmathis2016",
+UNSUPPORTED = {"test_disk-enabled_disk", "test_evolution-solar_like_bolmontmathis2016",
                "test_evolution-solar_like_galletbolmont2017"}
 SUPPORTED = [n for n in golden_names() if n not in UNSUPPORTED]
 STATE_KEYS = ("position", "velocity", "acceleration", "angular_momentum", "spin", "velocity_error",
@@ -48,7 +47,7 @@ def assert_same_state(gpu, ref, keys=STATE_KEYS, where=""):
 
 
 def test_fixtures_one_ensemble_bitwise(Ensemble):
-    """All 21 supported reference fixtures in one packed ensemble, run to
+    """All 23 supported reference fixtures in one packed ensemble, run to
     completion (199 steps): positions, velocities and accelerations equal the
     reference's particle_<i>.json bit for bit."""
     specs, expected = [], []
@@ -182,3 +181,48 @@ def test_large_ensemble_subset_properties(Ensemble, oracle):
     idx = np.concatenate([np.arange(spec.body_offset[s], spec.body_offset[s + 1]) for s in pick])
     sub_out = {k: (v[pick] if v.shape[0] == spec.n_systems else v[idx]) for k, v in out.items()}
     assert_same_state(sub_out, ref, where="subset")
+
+
+def gr_variant_ensemble():
+    """The uneven-size ensemble and the Jacobi binary-host ensemble, each under
+    every general-relativity implementation (Kidder, Anderson, Newhall,
+    disabled), interleaved so that device order differs from input order."""
+    parts = []
+    for impl in (abi.GR_KIDDER1995, abi.GR_ANDERSON1975, abi.GR_NEWHALL1983, abi.GR_DISABLED):
+        for s in (gen.config2(24, seed=11 + impl), gen.config3(8, seed=21 + impl)):
+            s.system["gr_implementation"][:] = impl
+            parts.append(s)
+    spec = EnsembleSpec.concat(parts)
+    return spec.subset(np.random.default_rng(5).permutation(spec.n_systems))
+
+
+@pytest.mark.parametrize("mapping,group_size,graph", [("packed", -1, True), ("packed", 0, False), ("plain", 0, False)])
+def test_gr_variants_against_oracle(Ensemble, oracle, mapping, group_size, graph):
+    """Anderson 1975 and Newhall 1983 (general_relativity.rs:461-895) run in
+    launches of their own; all four implementations in one ensemble agree
+    with the oracle bit for bit."""
+    spec = gr_variant_ensemble()
+    steps = 80
+    ref = oracle.run(spec, steps, threads=8)
+    log(f"GR variants mapping={mapping} G={group_size} graph={graph}: {spec.n_systems} systems x {steps} steps")
+    with Ensemble(spec, mapping=mapping, group_size=group_size, steps_per_launch=16, use_graph=graph) as ens:
+        ens.step(steps)
+        out = ens.get_state()
+    assert_same_state(out, ref, where=f"GR variants {mapping} G={group_size}")
+
+
+def test_gr_variants_operators_against_oracle(Ensemble, oracle):
+    spec = gr_variant_ensemble()
+    with Ensemble(spec) as ens:
+        e_acc, e_dl = ens.additional_effects()
+    r_eacc, r_edl = oracle.additional_effects(spec)
+    np.testing.assert_array_equal(e_acc, r_eacc)
+    np.testing.assert_array_equal(e_dl, r_edl)
+
+
+def test_adjoint_rejects_other_gr_variants(Ensemble):
+    spec = gen.config2(4, seed=3)
+    spec.system["gr_implementation"][:] = abi.GR_NEWHALL1983
+    with Ensemble(spec) as ens:
+        with pytest.raises(Exception, match="Kidder1995"):
+            ens.adjoint_record(10, 5)
