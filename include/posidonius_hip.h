@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define POS_ABI_VERSION 1
+#define POS_ABI_VERSION 2
 
 /* Largest body count per system.  The reference caps a universe at
  * MAX_PARTICLES = 10 (src/constants.rs:3); the uneven-size ensemble needs 12.
@@ -76,6 +76,8 @@ extern "C" {
 #define POS_ROLE_GR_CENTRAL      (1u << 4)
 #define POS_ROLE_GR_ORBITING     (1u << 5)
 #define POS_ROLE_WIND            (1u << 6)
+#define POS_ROLE_DISK_CENTRAL    (1u << 7)   /* DiskEffect::CentralBody (disk.rs:44-49) */
+#define POS_ROLE_DISK_ORBITING   (1u << 8)   /* DiskEffect::OrbitingBody                */
 
 /* ---- evolution models (src/effects/evolution.rs:9-17) ------------------ */
 #define POS_EVO_NON_EVOLVING              0
@@ -160,6 +162,17 @@ typedef struct pos_ensemble_desc {
     const double   *evolver_radius_of_gyration_2;
     const double   *evolver_love_number;
     const int32_t  *evolver_left_index;      /* [nb] Evolver.left_index              */
+
+    /* inverse tidal Q factor column of the same tables (optional; read for
+     * BolmontMathis2016, GalletBolmont2017 and LeconteChabrier2013 with
+     * dynamical tides, evolution.rs:487-513) */
+    const double   *evolver_inverse_tidal_q_factor;
+    /* the disk of each system's disk host (optional; required for systems
+     * with POS_EFFECT_DISK): 6 doubles per system, the host's DiskProperties
+     * (disk.rs:7-14): inner_edge_distance, outer_edge_distance (AU),
+     * lifetime (days), alpha, surface_density_normalization,
+     * mean_molecular_weight */
+    const double   *disk_properties;
 } pos_ensemble_desc;
 
 /*
@@ -271,9 +284,11 @@ int32_t pos_additional_effects(pos_ens *ens, double *out_acc, double *out_dangul
  * drops the record and sets POS_STATUS_HISTORY_OVERFLOW).
  * pos_history_drain copies the pending records out (input order: system,
  * then record, then body) and clears the buffer; *n_out receives the count.
- * The fields are those of the reference's history file; lag_angle and
- * migration_timescale are 0 (no evolving dissipation, no disk) and
- * denergy_dt (tides/common.rs:263-281) is evaluated at the snapshot state.
+ * The fields are those of the reference's history file: lag_angle is the
+ * body's evolving-dissipation lag angle at the snapshot (evolution.rs:548-567,
+ * 0 for other models), migration_timescale the disk migration timescale of
+ * its last disk evaluation (disk.rs:236, 0 without the disk) and denergy_dt
+ * (tides/common.rs:263-281) is evaluated at the snapshot state.
  */
 typedef struct pos_history_record {
     int32_t system;            /* input system index                      */

@@ -12,8 +12,17 @@
 
 #include <cmath>
 #include <cstdint>
+#include <quadmath.h>
 
 namespace oracle {
+
+// exp and x^y correctly rounded: the binary128 evaluation (113-bit
+// significand, error below one binary128 ulp) rounded to the nearest double.
+// The reference calls the platform libm here (f64::exp, f64::powf), whose
+// results are within 0.52 ulp; the device computes the correctly rounded
+// value in double-double arithmetic, and both sides round the same number.
+inline double exp_r(double x) { return (double)expq((__float128)x); }
+inline double pow_r(double x, double y) { return (double)powq((__float128)x, (__float128)y); }
 
 struct FlopCount {
     double v = 0.;
@@ -31,6 +40,8 @@ inline FlopCount operator-(FlopCount a) { return FlopCount(-a.v); }
 inline double val(const FlopCount &x) { return x.v; }
 inline FlopCount sqrt_r(const FlopCount &x) { ++FlopCount::count; return FlopCount(std::sqrt(x.v)); }
 inline FlopCount abs_r(const FlopCount &x) { return FlopCount(std::fabs(x.v)); }
+inline FlopCount exp_r(const FlopCount &x) { ++FlopCount::count; return FlopCount(exp_r(x.v)); }
+inline FlopCount pow_r(const FlopCount &x, double y) { ++FlopCount::count; return FlopCount(pow_r(x.v, y)); }
 
 }  // namespace oracle
 
@@ -83,5 +94,16 @@ template <int K> inline Dual<K> sqrt_r(const Dual<K> &x) {
     return r;
 }
 template <int K> inline Dual<K> abs_r(const Dual<K> &x) { return x.v < 0. ? -x : x; }
+template <int K> inline Dual<K> exp_r(const Dual<K> &x) {
+    Dual<K> r(exp_r(x.v));
+    for (int k = 0; k < K; ++k) r.d[k] = x.d[k] * r.v;
+    return r;
+}
+template <int K> inline Dual<K> pow_r(const Dual<K> &x, double y) {
+    Dual<K> r(pow_r(x.v, y));
+    const double h = y * r.v / x.v;
+    for (int k = 0; k < K; ++k) r.d[k] = x.d[k] * h;
+    return r;
+}
 
 }  // namespace oracle

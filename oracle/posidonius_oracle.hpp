@@ -12,8 +12,12 @@
 //   src/effects/tides/{common,constant_time_lag}.rs
 //   src/effects/rotational_flattening/{common,oblate_spheroid}.rs
 //   src/effects/general_relativity.rs (Kidder 1995)
+//   src/effects/general_relativity.rs (Anderson 1975, Newhall 1983)
 //   src/effects/wind.rs
-//   src/effects/evolution.rs:449-546 and src/tools.rs:840-907 (tables)
+//   src/effects/disk.rs
+//   src/effects/evolution.rs:449-567 and src/tools.rs:840-907 (tables,
+//   inverse tidal Q factor and lag angle of evolving dissipation)
+//   src/effects/tides/constant_time_lag.rs:20-165 (pair-dependent factors)
 // Data layout and decomposition are this project's own; what is kept alike
 // is the order of every floating-point operation, because the golden
 // fixtures are compared at 1e-14 absolute and the GPU path bitwise.  Rust's
@@ -29,7 +33,12 @@
 //     raise POS_STATUS_ITERATION_CAP;
 //   * a system must have every effect host equal to its most massive body,
 //     and when tides / flattening are on every other body must be an
-//     orbiting body of that effect.  Other layouts are rejected at load.
+//     orbiting body of that effect.  Other layouts are rejected at load;
+//   * evolving dissipation (BolmontMathis2016, GalletBolmont2017,
+//     LeconteChabrier2013 with dynamical tides) is accepted for the tidal
+//     host only; the pair-dependent factor map starts empty;
+//   * exp and powf are correctly rounded (oracle_types.hpp), where the
+//     reference's libm is within 0.52 ulp.
 #pragma once
 
 #include <cmath>
@@ -59,6 +68,10 @@ constexpr double R_SUN = 6.957e8 / AU;
 constexpr double G = G_SI / (AU * AU * AU) * M_SUN * (DAY * DAY);
 constexpr double K2 = G;
 constexpr double MAX_DISTANCE_2 = 100. * 100.;
+constexpr double SUN_DYN_FREQ_2 = K2 / (R_SUN * R_SUN * R_SUN);
+constexpr double SMOOTHING_FACTOR_DYN_TIDE_COROTATION = 1.0e-5 * 86400.;
+constexpr double BOLTZMANN_CONSTANT = 1.380649e-23 / (M_SUN * AU * AU) * DAY * DAY;
+constexpr double MASS_HYDROGEN_ATOM = 1.672162841e-27 / M_SUN;
 constexpr int WHFAST_NMAX_QUART = 64;
 constexpr int WHFAST_NMAX_NEWT = 32;
 constexpr int IMPLICIT_MIDPOINT_MIN_ITER = 3;
@@ -92,7 +105,7 @@ template <class R> inline V3<R> v3zero() { return V3<R>{R(0.), R(0.), R(0.)}; }
 // ---- Evolver (src/effects/evolution.rs:19-27, 449-513) --------------------
 struct Evolver {
     int type = POS_EVO_NON_EVOLVING;
-    const double *time = nullptr, *radius = nullptr, *rg2 = nullptr, *love = nullptr;
+    const double *time = nullptr, *radius = nullptr, *rg2 = nullptr, *love = nullptr, *itq = nullptr;
     int len = 0;
     int64_t left_index = 0;
 
@@ -142,6 +155,18 @@ struct Evolver {
         }
         return current;
     }
+    bool dynamical() const {
+        return type == POS_EVO_BOLMONT_MATHIS2016 || type == POS_EVO_GALLET_BOLMONT2017 ||
+               type == POS_EVO_LECONTE_CHABRIER2013_DYN;
+    }
+    // evolution.rs:487-513
+    template <class R> R eval_inverse_tidal_q_factor(const R &t) {
+        if (!dynamical()) return R(0.);
+        int i0 = idx(), left = 0;
+        R r = interpolate(t, time + i0, itq + i0, len - i0, &left);
+        left_index += left;
+        return r;
+    }
     template <class R> R eval_love(const R &t, const R &current) {
         if (type == POS_EVO_LECONTE_CHABRIER2013 || type == POS_EVO_LECONTE_CHABRIER2013_DYN) {
             int i0 = idx(), left = 0;
@@ -165,6 +190,9 @@ template <class R> struct Body {
     uint32_t roles;
     // parameters
     R sigma;      // scaled CTL dissipation factor (tides/common.rs:104-112)
+    R sigma_scale, sigma_factor;   // its two factors (ConstantTimeLagParameters)
+    R lag;        // lag angle of evolving dissipation (evolution.rs:548-567)
+    R sigma_host; // the host's factor in this body's pair (constant_time_lag.rs:20-94, 196-204)
     R k2;         // CTL love number
     R k2f;        // oblate-spheroid (fluid) love number
     R wind_k, wind_sat, wind_sat2;
@@ -179,6 +207,9 @@ template <class R> struct Body {
     V3<R> gr_acc, gr_dLdt;
     // wind
     V3<R> wind_dLdt;
+    // disk (disk.rs:17-34)
+    R disk_migration;
+    V3<R> disk_acc;
 };
 
 template <class R> struct Alt {   // AlternativeCoordinates (whfast.rs:122-130)
@@ -188,7 +219,8 @@ template <class R> struct Alt {   // AlternativeCoordinates (whfast.rs:122-130)
 
 template <class R> struct System {
     int n = 0, host = 0, coord = POS_COORD_DEMOCRATIC_HELIOCENTRIC, gr_impl = POS_GR_DISABLED;
-    bool tides = false, rf = false, gr = false, wind = false, evolution = false;
+    bool tides = false, rf = false, gr = false, wind = false, evolution = false, disk = false;
+    double disk_props[6] = {0., 0., 0., 0., 0., 0.};   // DiskProperties of the disk host
     R dt, half_dt, t, time_limit;
     double hist_period = std::numeric_limits<double>::infinity();
     double last_hist = -1.;
@@ -219,7 +251,11 @@ inline bool load_system(const pos_ensemble_desc *d, int s, System<R> &S, std::st
     S.gr = eff & POS_EFFECT_GENERAL_RELATIVITY;
     S.wind = eff & POS_EFFECT_WIND;
     S.evolution = eff & POS_EFFECT_EVOLUTION;
-    if (eff & POS_EFFECT_DISK) { if (err) *err = "disk effect not supported"; return false; }
+    S.disk = eff & POS_EFFECT_DISK;
+    if (S.disk) {
+        if (!d->disk_properties) { if (err) *err = "disk effect without disk properties"; return false; }
+        for (int q = 0; q < 6; ++q) S.disk_props[q] = d->disk_properties[6 * s + q];
+    }
     S.coord = d->coordinates[s];
     S.gr_impl = d->gr_implementation[s];
     if (S.gr && S.gr_impl == POS_GR_DISABLED) S.gr = false;
@@ -255,7 +291,9 @@ inline bool load_system(const pos_ensemble_desc *d, int s, System<R> &S, std::st
         if (d->heliocentric_velocity)
             P.hv = V3<R>{R(d->heliocentric_velocity[3 * b]), R(d->heliocentric_velocity[3 * b + 1]), R(d->heliocentric_velocity[3 * b + 2])};
         P.roles = d->roles[b];
-        P.sigma = R(d->tides_dissipation_factor_scale[b]) * R(d->tides_dissipation_factor[b]);
+        P.sigma_scale = R(d->tides_dissipation_factor_scale[b]);
+        P.sigma_factor = R(d->tides_dissipation_factor[b]);
+        P.sigma = P.sigma_scale * P.sigma_factor;
         P.k2 = R(d->tides_love_number[b]);
         P.k2f = R(d->flattening_love_number[b]);
         P.wind_k = R(d->wind_k_factor ? d->wind_k_factor[b] : 0.);
@@ -269,9 +307,8 @@ inline bool load_system(const pos_ensemble_desc *d, int s, System<R> &S, std::st
         E = Evolver();
         if (d->evolution_type) {
             E.type = d->evolution_type[b];
-            if (E.type == POS_EVO_BOLMONT_MATHIS2016 || E.type == POS_EVO_GALLET_BOLMONT2017 ||
-                E.type == POS_EVO_LECONTE_CHABRIER2013_DYN) {
-                if (err) *err = "evolving dissipation models are not supported";
+            if (E.dynamical() && i != S.host) {
+                if (err) *err = "evolving dissipation of an orbiting body is not supported";
                 return false;
             }
             if (E.type != POS_EVO_NON_EVOLVING) {
@@ -282,6 +319,10 @@ inline bool load_system(const pos_ensemble_desc *d, int s, System<R> &S, std::st
                 E.radius = d->evolver_radius + o;
                 E.rg2 = d->evolver_radius_of_gyration_2 + o;
                 E.love = d->evolver_love_number + o;
+                if (E.dynamical()) {
+                    if (!d->evolver_inverse_tidal_q_factor) { if (err) *err = "evolving dissipation without a Q-factor table"; return false; }
+                    E.itq = d->evolver_inverse_tidal_q_factor + o;
+                }
                 E.left_index = d->evolver_left_index ? d->evolver_left_index[b] : 0;
             }
         }
@@ -291,10 +332,11 @@ inline bool load_system(const pos_ensemble_desc *d, int s, System<R> &S, std::st
     if (S.tides && !(hr & POS_ROLE_TIDES_CENTRAL)) { if (err) *err = "tidal host is not the most massive body"; return false; }
     if (S.rf && !(hr & POS_ROLE_FLAT_CENTRAL)) { if (err) *err = "flattening host is not the most massive body"; return false; }
     if (S.gr && !(hr & POS_ROLE_GR_CENTRAL)) { if (err) *err = "GR host is not the most massive body"; return false; }
+    if (S.disk && !(hr & POS_ROLE_DISK_CENTRAL)) { if (err) *err = "disk host is not the most massive body"; return false; }
     for (int i = 0; i < n; ++i) {
         if (i == S.host) continue;
         const uint32_t r = S.p[i].roles;
-        if (r & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_FLAT_CENTRAL | POS_ROLE_GR_CENTRAL)) {
+        if (r & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_FLAT_CENTRAL | POS_ROLE_GR_CENTRAL | POS_ROLE_DISK_CENTRAL)) {
             if (err) *err = "a second central body";
             return false;
         }
@@ -349,11 +391,26 @@ template <class R> inline void calculate_spin(System<R> &S) {
     }
 }
 
-// universe.rs:305-316 (spin-dependent evolution only sets lag angles of the
-// evolving-dissipation models, which are rejected at load)
+// evolution.rs:548-567: lag angles of the evolving-dissipation models
+template <class R> inline void evolve_spin_dependent(System<R> &S, const R &t) {
+    for (int i = 0; i < S.n; ++i) {
+        Body<R> &P = S.p[i];
+        Evolver &E = S.ev[i];
+        if (E.dynamical()) {
+            const R inverse_tidal_q_factor = E.eval_inverse_tidal_q_factor(t);
+            const R epsilon_squared = P.spin2 / R(SUN_DYN_FREQ_2);
+            P.lag = R(3.0) * epsilon_squared * inverse_tidal_q_factor / R(4.0);
+        } else {
+            P.lag = R(0.);
+        }
+    }
+}
+
+// universe.rs:305-316
 template <class R> inline void calculate_spin_and_evolving_quantities(System<R> &S, const R &t, bool evolution) {
     if (evolution && S.evolution) evolve_non_spin_dependent(S, t);
     calculate_spin(S);
+    if (evolution && S.evolution) evolve_spin_dependent(S, t);
 }
 
 template <class R> inline void initialize_physical_values(System<R> &S) {
@@ -429,14 +486,59 @@ template <class R> inline void gr_initialize(System<R> &S) {
     }
 }
 
-// constant_time_lag.rs:206-264 (pair-dependent factors are a no-op here)
+// tools.rs:251-285
+template <class R> inline void perihelion_distance_and_eccentricity(const R &gm, const V3<R> &x, const V3<R> &u, R &q, R &e) {
+    const R hx = x.y * u.z - x.z * u.y;
+    const R hy = x.z * u.x - x.x * u.z;
+    const R hz = x.x * u.y - x.y * u.x;
+    const R h2 = hx * hx + hy * hy + hz * hz;   // powf(2.) is the exact square
+    const R v2 = u.x * u.x + u.y * u.y + u.z * u.z;
+    const R r = sqrt_r(x.x * x.x + x.y * x.y + x.z * x.z);
+    const R s = h2 / gm;
+    const R temp = R(1.) + s * (v2 / gm - R(2.) / r);
+    e = val(temp) <= 0. ? R(0.) : sqrt_r(temp);
+    q = s / (R(1.) + e);
+}
+
+// constant_time_lag.rs:20-94: the host's dissipation factor as seen by each
+// orbiting body.  With a host of an evolving-dissipation model whose dynamical
+// tide is excited by the body's mean motion, the pair value; otherwise the
+// host's own scaled factor (the map entry is removed, 196-204).  The
+// host-dependent half (96-165) applies to orbiting bodies of those models,
+// which are rejected at load.
+template <class R> inline void pair_dependent_dissipation_factors(System<R> &S) {
+    const Body<R> &H = S.p[S.host];
+    for (int i = 0; i < S.n; ++i) {
+        if (i == S.host) continue;
+        S.p[i].sigma_host = H.sigma;
+    }
+    if (!S.ev[S.host].dynamical()) return;
+    const R host_norm_spin = sqrt_r(H.spin2);
+    for (int i = 0; i < S.n; ++i) {
+        if (i == S.host) continue;
+        Body<R> &P = S.p[i];
+        if (!(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
+        const R gm = H.mg + P.mg;
+        R q, e;
+        perihelion_distance_and_eccentricity(gm, P.hx, P.hv, q, e);
+        const R mean_motion = sqrt_r(gm) * pow_r(q / (R(1.0) - e), -1.5);
+        R half = abs_r(host_norm_spin - mean_motion);
+        if (val(half) < val(host_norm_spin)) {
+            if (val(half) < SMOOTHING_FACTOR_DYN_TIDE_COROTATION) half = R(SMOOTHING_FACTOR_DYN_TIDE_COROTATION);
+            const R inverse_of_half = R(1.) / half;
+            P.sigma_host = H.sigma_scale * (R(2.0 * K2) / (R(3.0) * pw5(H.radius)) * H.lag * inverse_of_half + H.sigma_factor);
+        }
+    }
+}
+
+// constant_time_lag.rs:206-264
 template <class R> inline void tides_orthogonal(System<R> &S) {
     Body<R> &H = S.p[S.host];
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
         R d7 = pw7(P.hdist);
-        P.td_orth_star = R(4.5) * pw2(P.m) * pw10(H.radius) * H.sigma / d7;
+        P.td_orth_star = R(4.5) * pw2(P.m) * pw10(H.radius) * P.sigma_host / d7;
     }
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
@@ -472,7 +574,7 @@ template <class R> inline void tides_radial(System<R> &S) {
         const R pm2 = P.m * P.m;
         R conservative = R(-3.0 * K2) / pw7(P.hdist) * (pm2 * pw5(H.radius) * H.k2 + host_mass_2 * pw5(P.radius) * P.k2);
         R factor1 = R(-13.5) * P.hvr / pw8(P.hdist);
-        R term1 = pm2 * pw10(H.radius) * H.sigma;
+        R term1 = pm2 * pw10(H.radius) * P.sigma_host;
         R term2 = host_mass_2 * pw10(P.radius) * P.sigma;
         P.td_diss_pm = factor1 * term2;
         R dissipative = P.td_diss_pm + factor1 * term1;
@@ -949,6 +1051,58 @@ template <class R> inline void wind_factor(System<R> &S) {
     }
 }
 
+// disk.rs:155-194: the host's DiskProperties methods
+template <class R>
+inline R disk_migration_timescale(const double *dp, const R &time, const R &host_mass, const R &host_mass_g,
+                                  const R &mass, const R &distance, const R &sma) {
+    const double inner = dp[0], outer = dp[1], lifetime = dp[2], alpha = dp[3], sdn = dp[4], mmw = dp[5];
+    // gas radial velocity (disk.rs:222-226) from the viscosity (212-220)
+    const R keplerian_frequency = sqrt_r(host_mass_g / pw3(distance));
+    const R temperature = R(280.0) * pow_r(distance, -0.5) * host_mass;
+    const R speed_of_sound_2 = R(BOLTZMANN_CONSTANT) * temperature / (R(mmw) * R(MASS_HYDROGEN_ATOM));
+    const R viscosity = R(alpha) * speed_of_sound_2 / keplerian_frequency;
+    const R gas_radial_velocity = R(-3.0) * viscosity / (R(2.0) * sma);
+    // surface density (disk.rs:196-205); powi(-1) is 1/x
+    const R initial = R(sdn) * (R(1.) / distance) * exp_r(R(-1.0) * distance / R(outer)) *
+                      (R(1.0) - sqrt_r(R(inner) / distance));
+    R surface_density = R(0.0);
+    if (val(distance) > inner) surface_density = initial * exp_r(-time / R(lifetime));
+    const R ratio = R(2.0) * surface_density * pw2(sma) / mass;
+    const R a_dot = gas_radial_velocity * (std::isnan(val(ratio)) || val(ratio) > 1.0 ? R(1.0) : ratio);
+    return -sma / a_dot;
+}
+
+// disk.rs:86-99 (initialize) and 228-268 (Alibert et al. 2013 forces)
+template <class R> inline void disk_acceleration(System<R> &S, const R &t) {
+    Body<R> &H = S.p[S.host];
+    const R factor2 = R(1.) / H.m;
+    V3<R> sum = v3zero<R>();
+    for (int i = 0; i < S.n; ++i) S.p[i].disk_acc = v3zero<R>();
+    for (int i = 0; i < S.n; ++i) {
+        if (i == S.host) continue;
+        Body<R> &P = S.p[i];
+        if (P.roles & POS_ROLE_DISK_ORBITING) {
+            const R d = P.hdist;
+            const R sma = (P.mg + H.mg) * d / (R(2.0) * (P.mg + H.mg) - d * P.hnv2);
+            P.disk_migration = disk_migration_timescale(S.disk_props, t, H.m, H.mg, P.m, d, sma);
+            const R factor1 = R(1.) / P.m;
+            const R factor_migration = -P.m / P.disk_migration;
+            const R ecc_timescale = R(0.1) * P.disk_migration;
+            const R factor_ecc = -P.m * R(2.0) / ecc_timescale;
+            const R inc_timescale = ecc_timescale;
+            const R factor_inc = -P.m * R(2.0) / inc_timescale;
+            const R sp = R(1.0) / (d * d) * (P.hx.x * P.hv.x + P.hx.y * P.hv.y + P.hx.z * P.hv.z);
+            const V3<R> Fm{factor_migration * P.hv.x, factor_migration * P.hv.y, factor_migration * P.hv.z};
+            const V3<R> Fe{factor_ecc * sp * P.hx.x, factor_ecc * sp * P.hx.y, factor_ecc * sp * P.hx.z};
+            const V3<R> Fi{R(0.0), R(0.0), factor_inc * P.hv.z};
+            const V3<R> F{Fm.x + Fe.x + Fi.x, Fm.y + Fe.y + Fi.y, Fm.z + Fe.z + Fi.z};
+            sum.x = sum.x + F.x; sum.y = sum.y + F.y; sum.z = sum.z + F.z;
+            P.disk_acc = V3<R>{factor1 * F.x, factor1 * F.y, factor1 * F.z};
+        }
+        H.disk_acc = V3<R>{R(-1.0) * factor2 * sum.x, R(-1.0) * factor2 * sum.y, R(-1.0) * factor2 * sum.z};
+    }
+}
+
 // universe.rs:353-426 + 428-538 + 540-614
 template <class R>
 inline void calculate_additional_effects(System<R> &S, const R &t, bool evolution, bool dangular_momentum_dt, bool accelerations) {
@@ -969,7 +1123,8 @@ inline void calculate_additional_effects(System<R> &S, const R &t, bool evolutio
     if (dangular_momentum_dt && S.wind) wind_factor(S);
 
     if (S.tides || S.rf) {
-        if ((dangular_momentum_dt && (S.tides || S.rf)) || (accelerations && (S.tides || S.rf || S.gr))) {
+        if ((dangular_momentum_dt && (S.tides || S.rf)) || (accelerations && (S.tides || S.disk || S.rf || S.gr))) {
+            if (S.tides) pair_dependent_dissipation_factors(S);
             if (S.tides) tides_orthogonal(S);
             if (S.rf) flattening_orthogonal(S);
             if (accelerations && (S.tides || S.rf)) {
@@ -989,6 +1144,7 @@ inline void calculate_additional_effects(System<R> &S, const R &t, bool evolutio
     }
     if (accelerations && S.gr && S.gr_impl == POS_GR_ANDERSON1975) anderson1975(S);
     if (accelerations && S.gr && S.gr_impl == POS_GR_NEWHALL1983) newhall1983(S);
+    if (accelerations && S.disk) disk_acceleration(S, t);
     if (dangular_momentum_dt && (S.tides || S.rf || (S.gr && S.gr_impl == POS_GR_KIDDER1995) || S.wind)) {
         for (int i = 0; i < S.n; ++i) {
             Body<R> &P = S.p[i];
@@ -1001,6 +1157,7 @@ inline void calculate_additional_effects(System<R> &S, const R &t, bool evolutio
         for (int i = 0; i < S.n; ++i) {
             Body<R> &P = S.p[i];
             if (S.tides) { P.a_add.x = P.a_add.x + P.td_acc.x; P.a_add.y = P.a_add.y + P.td_acc.y; P.a_add.z = P.a_add.z + P.td_acc.z; }
+            if (S.disk) { P.a_add.x = P.a_add.x + P.disk_acc.x; P.a_add.y = P.a_add.y + P.disk_acc.y; P.a_add.z = P.a_add.z + P.disk_acc.z; }
             if (S.rf) { P.a_add.x = P.a_add.x + P.rf_acc.x; P.a_add.y = P.a_add.y + P.rf_acc.y; P.a_add.z = P.a_add.z + P.rf_acc.z; }
             if (S.gr) { P.a_add.x = P.a_add.x + P.gr_acc.x; P.a_add.y = P.a_add.y + P.gr_acc.y; P.a_add.z = P.a_add.z + P.gr_acc.z; }
         }

@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_BODIES = 16
 
 OK = 0
@@ -48,6 +48,8 @@ ROLE_FLAT_ORBITING = 1 << 3
 ROLE_GR_CENTRAL = 1 << 4
 ROLE_GR_ORBITING = 1 << 5
 ROLE_WIND = 1 << 6
+ROLE_DISK_CENTRAL = 1 << 7
+ROLE_DISK_ORBITING = 1 << 8
 
 EVO_NON_EVOLVING = 0
 EVO_LECONTE2011 = 1
@@ -118,6 +120,8 @@ class EnsembleDesc(C.Structure):
         ("evolver_radius_of_gyration_2", _pd),
         ("evolver_love_number", _pd),
         ("evolver_left_index", _pi32),
+        ("evolver_inverse_tidal_q_factor", _pd),
+        ("disk_properties", _pd),
     ]
 
 

@@ -38,7 +38,14 @@ BODY_FIELDS = {
     "angular_momentum_error": (np.float64, 3), "evolution_type": (np.int32, 1),
     "evolver_left_index": (np.int32, 1),
 }
-TABLE_FIELDS = ("evolver_time", "evolver_radius", "evolver_radius_of_gyration_2", "evolver_love_number")
+TABLE_FIELDS = ("evolver_time", "evolver_radius", "evolver_radius_of_gyration_2", "evolver_love_number",
+                "evolver_inverse_tidal_q_factor")
+# Per-system fields wider than one value: DiskProperties of the disk host
+# (inner_edge_distance, outer_edge_distance, lifetime, alpha,
+# surface_density_normalization, mean_molecular_weight; disk.rs:7-14)
+SYSTEM_WIDE_FIELDS = {"disk_properties": 6}
+DISK_KEYS = ("inner_edge_distance", "outer_edge_distance", "lifetime", "alpha", "surface_density_normalization",
+             "mean_molecular_weight")
 
 _EVOLUTION_CODES = {
     "NonEvolving": abi.EVO_NON_EVOLVING, "Leconte2011": abi.EVO_LECONTE2011,
@@ -81,6 +88,13 @@ class EnsembleSpec:
         for k, dt in SYSTEM_FIELDS.items():
             parts = [s.system.get(k) for s in specs]
             out.system[k] = None if any(p is None for p in parts) else np.concatenate(parts).astype(dt)
+        for k, w in SYSTEM_WIDE_FIELDS.items():
+            parts = [s.system.get(k) for s in specs]
+            if all(p is None for p in parts):
+                out.system[k] = None
+            else:
+                out.system[k] = np.concatenate([np.zeros((s.n_systems, w)) if p is None else np.asarray(p).reshape(-1, w)
+                                                for s, p in zip(specs, parts)]).astype(np.float64)
         for k, (dt, w) in BODY_FIELDS.items():
             parts = [s.body.get(k) for s in specs]
             if all(p is None for p in parts):
@@ -101,8 +115,9 @@ class EnsembleSpec:
                     continue
                 base = eo[-1]
                 eo.extend((base + s.evolver_offset[1:]).tolist())
+                rows = int(s.evolver_offset[-1])
                 for k in TABLE_FIELDS:
-                    tabs[k].append(s.tables[k])
+                    tabs[k].append(s.tables[k] if k in s.tables else np.zeros(rows))
             out.evolver_offset = np.asarray(eo, dtype=np.int32)
             out.tables = {k: (np.concatenate(v) if v else np.zeros(0)).astype(np.float64) for k, v in tabs.items()}
         out.names = sum((s.names or [""] * s.n_systems for s in specs), [])
@@ -150,12 +165,15 @@ class EnsembleSpec:
                      np.int64: abi.C.c_int64}
         for k, dt in SYSTEM_FIELDS.items():
             setattr(d, k, abi.ptr(arr(self.system.get(k), dt), ctypes_of[dt]))
+        for k in SYSTEM_WIDE_FIELDS:
+            setattr(d, k, abi.ptr(arr(self.system.get(k), np.float64), abi.C.c_double))
         for k, (dt, _w) in BODY_FIELDS.items():
             setattr(d, k, abi.ptr(arr(self.body.get(k), dt), ctypes_of[dt]))
         if self.evolver_offset is not None:
             d.evolver_offset = abi.ptr(arr(self.evolver_offset, np.int32), abi.C.c_int32)
+            rows = int(self.evolver_offset[-1])
             for k in TABLE_FIELDS:
-                setattr(d, k, abi.ptr(arr(self.tables[k], np.float64), abi.C.c_double))
+                setattr(d, k, abi.ptr(arr(self.tables.get(k, np.zeros(rows)), np.float64), abi.C.c_double))
         return _DescHolder(d, keep)
 
 
@@ -264,6 +282,14 @@ def read_case(path_or_dict) -> EnsembleSpec:
             roles |= abi.ROLE_GR_CENTRAL
         elif tag == "OrbitingBody":
             roles |= abi.ROLE_GR_ORBITING
+        disk = p.get("disk")
+        if disk is not None:
+            tag, payload = _enum(disk["effect"])
+            if tag == "CentralBody":
+                roles |= abi.ROLE_DISK_CENTRAL
+                spec.system["disk_properties"] = np.array([[float(payload[k]) for k in DISK_KEYS]])
+            elif tag == "OrbitingBody":
+                roles |= abi.ROLE_DISK_ORBITING
         wk = ws = 0.0
         if p["wind"]["effect"] == "Interaction":
             roles |= abi.ROLE_WIND
@@ -286,10 +312,13 @@ def read_case(path_or_dict) -> EnsembleSpec:
             tabs["evolver_radius"].append(np.asarray(ev["radius"], dtype=np.float64))
             rg = ev.get("radius_of_gyration_2") or []
             lv = ev.get("love_number") or []
+            iq = ev.get("inverse_tidal_q_factor") or []
             tabs["evolver_radius_of_gyration_2"].append(
                 np.asarray(rg, dtype=np.float64) if len(rg) == len(t) else np.zeros(len(t)))
             tabs["evolver_love_number"].append(
                 np.asarray(lv, dtype=np.float64) if len(lv) == len(t) else np.zeros(len(t)))
+            tabs["evolver_inverse_tidal_q_factor"].append(
+                np.asarray(iq, dtype=np.float64) if len(iq) == len(t) else np.zeros(len(t)))
             body["evolver_left_index"].append(int(ev.get("left_index", 0)))
             eo.append(eo[-1] + len(t))
         else:

@@ -67,6 +67,10 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
         ((eff & POS_EFFECT_GENERAL_RELATIVITY) && eff_gr_variant(eff) == GRV_KIDDER))
         f |= F_SPIN;
     if (roles & POS_ROLE_WIND) f |= F_WIND_ROLE;
+    if (eff & POS_EFFECT_DISK) f |= F_DISK;
+    if (roles & POS_ROLE_DISK_ORBITING) f |= F_DISK_ROLE;
+    if (evo_dynamical(B.evo)) f |= F_DYN;
+    if (gvalid && evo_dynamical(E.evo_type[b0 + B.host])) f |= F_DYN_HOST;
     if (B.evo == POS_EVO_LECONTE_CHABRIER2013 && (roles & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_TIDES_ORBITING)))
         f |= F_LC_LOVE;
     B.flags = f;
@@ -144,11 +148,11 @@ __device__ inline void store_lane(const DevEnsemble &E, const WaveLds &lds, cons
 
 constexpr uint32_t FROZEN = POS_STATUS_COMPLETED | POS_STATUS_FATAL;
 
-// K steps of every system of waves [wave_begin, wave_end), all of GR variant
-// GRV (general relativity off or Kidder 1995 for GRV_KIDDER); the grid may be
-// smaller than the number of waves, in which case each workgroup walks the
-// table with a grid stride (persistent launch).
-template <int GRV>
+// K steps of every system of waves [wave_begin, wave_end), all of kernel
+// variant V (GR implementation, with or without the extended effects); the
+// grid may be smaller than the number of waves, in which case each workgroup
+// walks the table with a grid stride (persistent launch).
+template <int V>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POS_WAVES_PER_EU)))
 void whfast_steps_kernel(DevEnsemble E, int n_steps, int wave_begin, int wave_end) {
     __shared__ WaveLds lds;
@@ -158,7 +162,7 @@ void whfast_steps_kernel(DevEnsemble E, int n_steps, int wave_begin, int wave_en
         const TapeRef none{nullptr, 0};
         for (int step = 0; step < n_steps; ++step) {
             if (B.status & FROZEN) break;   // uniform within the group
-            iterate<false, GRV>(E, lds, B, none, 0);
+            iterate<false, V>(E, lds, B, none, 0);
         }
         store_lane(E, lds, B);
         __syncthreads();
@@ -208,6 +212,7 @@ __global__ __launch_bounds__(64) void init_kernel(DevEnsemble E) {
     if (B.t == 0.) {
         if (B.flags & F_EVOLUTION) evolve_body(E, lds, B, B.t);
         own_spin(lds, B);
+        if (B.flags & F_EVOLUTION) evolve_spin_dependent(E, lds, B, B.t);
         merge_status(lds, B);
         store_lane(E, lds, B);
     }
@@ -230,7 +235,7 @@ __global__ __launch_bounds__(64) void gravity_kernel(DevEnsemble E, double *out_
     }
 }
 
-template <int GRV>
+template <int V>
 __global__ __launch_bounds__(64) void effects_kernel(DevEnsemble E, double *out_acc, double *out_dLdt, int wave_begin,
                                                      int wave_end) {
     __shared__ WaveLds lds;
@@ -239,7 +244,7 @@ __global__ __launch_bounds__(64) void effects_kernel(DevEnsemble E, double *out_
     Lane B;
     load_lane(E, lds, B, wave);
     double a[3], dL[3];
-    additional_effects<GRV>(E, lds, B, false, a, dL);
+    additional_effects<V>(E, lds, B, false, a, dL);
     const int nb = E.n_bodies;
     if (B.valid()) {
 #pragma unroll
@@ -288,8 +293,8 @@ struct pos_ens {
     std::vector<int> n_per_sys_dev;
     // launch tables per mapping
     std::vector<int> wave_first, wave_count, wave_group;
-    int variant_waves[3][2] = {{0, 0}, {0, 0}, {0, 0}};   // [GR variant] -> wave range [begin, end)
-    std::vector<int> variant_dev;                          // GR variant of each device system
+    int variant_waves[N_VARIANTS][2] = {};   // [kernel variant] -> wave range [begin, end)
+    std::vector<int> variant_dev;           // kernel variant of each device system
     int *d_wave_first = nullptr, *d_wave_count = nullptr, *d_wave_group = nullptr;   // capacity n_systems
     double *d_op_a = nullptr, *d_op_l = nullptr;   // operator outputs, [3][n_bodies]
     uint32_t *d_op_st = nullptr;                    // operator status, [n_systems]
@@ -407,7 +412,7 @@ int build_launch(pos_ens *h, const pos_options &o) {
     std::reverse(h->wave_group.begin(), h->wave_group.end());
     const int nw = (int)h->wave_first.size();
     // device order groups the variants, so each one covers a contiguous range
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < N_VARIANTS; ++v) {
         h->variant_waves[v][0] = nw;
         h->variant_waves[v][1] = 0;
     }
@@ -438,12 +443,30 @@ int build_launch(pos_ens *h, const pos_options &o) {
     return POS_OK;
 }
 
-// GR variant a system's steps are compiled for (GRV_KIDDER also when GR is off)
-int gr_variant(uint32_t effects, int implementation) {
-    if (!(effects & POS_EFFECT_GENERAL_RELATIVITY)) return GRV_KIDDER;
-    if (implementation == POS_GR_ANDERSON1975) return GRV_ANDERSON;
-    if (implementation == POS_GR_NEWHALL1983) return GRV_NEWHALL;
-    return GRV_KIDDER;
+// Kernel variant a system's steps are compiled for: its GR implementation
+// (GRV_KIDDER also when GR is off) plus VAR_EXT with the disk effect or an
+// evolving-dissipation host.
+int system_variant(const pos_ensemble_desc *d, int s, int host) {
+    const uint32_t effects = d->effects[s];
+    const int implementation = d->gr_implementation[s];
+    int v = GRV_KIDDER;
+    if (effects & POS_EFFECT_GENERAL_RELATIVITY) {
+        if (implementation == POS_GR_ANDERSON1975) v = GRV_ANDERSON;
+        if (implementation == POS_GR_NEWHALL1983) v = GRV_NEWHALL;
+    }
+    const bool dyn_host = d->evolution_type && evo_dynamical(d->evolution_type[d->body_offset[s] + host]);
+    if ((effects & POS_EFFECT_DISK) || dyn_host) v |= VAR_EXT;
+    return v;
+}
+
+// most massive body of system s (strictly greater keeps the first maximum)
+int most_massive(const pos_ensemble_desc *d, int s) {
+    const int b0 = d->body_offset[s], n = d->body_offset[s + 1] - b0;
+    int mm = 0;
+    double mmax = 0.;
+    for (int i = 0; i < n; ++i)
+        if (d->mass[b0 + i] > mmax) { mmax = d->mass[b0 + i]; mm = i; }
+    return mm;
 }
 
 int validate(const pos_ensemble_desc *d) {
@@ -462,7 +485,8 @@ int validate(const pos_ensemble_desc *d) {
         if (n < 2 || n > POS_MAX_BODIES)
             return fail(POS_E_LIMIT, where + "body count " + std::to_string(n) + " outside [2, " + std::to_string(POS_MAX_BODIES) + "]");
         const uint32_t eff = d->effects[s];
-        if (eff & POS_EFFECT_DISK) return fail(POS_E_UNSUPPORTED, where + "disk effect is not supported");
+        if ((eff & POS_EFFECT_DISK) && !d->disk_properties)
+            return fail(POS_E_INVALID, where + "the disk effect needs disk_properties");
         if (d->coordinates[s] < 0 || d->coordinates[s] > 2) return fail(POS_E_INVALID, where + "bad coordinates");
         if ((eff & POS_EFFECT_GENERAL_RELATIVITY) &&
             (d->gr_implementation[s] < POS_GR_KIDDER1995 || d->gr_implementation[s] > POS_GR_DISABLED))
@@ -481,10 +505,12 @@ int validate(const pos_ensemble_desc *d) {
             return fail(POS_E_UNSUPPORTED, where + "the flattening host must be the most massive body");
         if (gr_on && !(hr & POS_ROLE_GR_CENTRAL))
             return fail(POS_E_UNSUPPORTED, where + "the GR host must be the most massive body");
+        if ((eff & POS_EFFECT_DISK) && !(hr & POS_ROLE_DISK_CENTRAL))
+            return fail(POS_E_UNSUPPORTED, where + "the disk host must be the most massive body");
         for (int i = 0; i < n; ++i) {
             if (i == mm) continue;
             const uint32_t r = d->roles[b0 + i];
-            if (r & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_FLAT_CENTRAL | POS_ROLE_GR_CENTRAL))
+            if (r & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_FLAT_CENTRAL | POS_ROLE_GR_CENTRAL | POS_ROLE_DISK_CENTRAL))
                 return fail(POS_E_UNSUPPORTED, where + "only one central body per system");
             if ((eff & POS_EFFECT_TIDES) && !(r & POS_ROLE_TIDES_ORBITING))
                 return fail(POS_E_UNSUPPORTED, where + "with tides on every other body must be a tidal orbiting body");
@@ -502,8 +528,12 @@ int validate(const pos_ensemble_desc *d) {
         if (d->evolution_type) {
             for (int i = 0; i < n; ++i) {
                 const int et = d->evolution_type[b0 + i];
-                if (et < POS_EVO_NON_EVOLVING || et > POS_EVO_LECONTE_CHABRIER2013)
-                    return fail(POS_E_UNSUPPORTED, where + "evolving-dissipation models are not supported");
+                if (et < POS_EVO_NON_EVOLVING || et > POS_EVO_GALLET_BOLMONT2017)
+                    return fail(POS_E_INVALID, where + "bad evolution type");
+                if (evo_dynamical(et) && i != mm)
+                    return fail(POS_E_UNSUPPORTED, where + "evolving dissipation of an orbiting body is not supported");
+                if (evo_dynamical(et) && !d->evolver_inverse_tidal_q_factor)
+                    return fail(POS_E_INVALID, where + "evolving dissipation needs evolver_inverse_tidal_q_factor");
                 if (et != POS_EVO_NON_EVOLVING &&
                     (!d->evolver_offset || d->evolver_offset[b0 + i + 1] - d->evolver_offset[b0 + i] <= 0))
                     return fail(POS_E_INVALID, where + "evolving body without a table");
@@ -558,18 +588,25 @@ int adjoint_copy_state(pos_ens *h, char *slot, bool save) {
 
 constexpr int GRAPH_LAUNCHES = 4;   // kernel launches per captured graph
 
-// One launch per GR variant present; the persistent grid size caps each.
+template <int V> void launch_steps_variant(pos_ens *h, int grid, int n_steps, int w0, int w1) {
+    hipLaunchKernelGGL(whfast_steps_kernel<V>, dim3(grid), dim3(64), 0, h->stream, h->E, n_steps, w0, w1);
+}
+
+// One launch per kernel variant present; the persistent grid size caps each.
 int launch_steps(pos_ens *h, int n_steps) {
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < N_VARIANTS; ++v) {
         const int w0 = h->variant_waves[v][0], w1 = h->variant_waves[v][1];
         if (w1 <= w0) continue;
         const int grid = std::min(h->grid, w1 - w0);
-        if (v == GRV_KIDDER)
-            hipLaunchKernelGGL(whfast_steps_kernel<GRV_KIDDER>, dim3(grid), dim3(64), 0, h->stream, h->E, n_steps, w0, w1);
-        else if (v == GRV_ANDERSON)
-            hipLaunchKernelGGL(whfast_steps_kernel<GRV_ANDERSON>, dim3(grid), dim3(64), 0, h->stream, h->E, n_steps, w0, w1);
-        else
-            hipLaunchKernelGGL(whfast_steps_kernel<GRV_NEWHALL>, dim3(grid), dim3(64), 0, h->stream, h->E, n_steps, w0, w1);
+        switch (v) {
+            case 0: launch_steps_variant<0>(h, grid, n_steps, w0, w1); break;
+            case 1: launch_steps_variant<1>(h, grid, n_steps, w0, w1); break;
+            case 2: launch_steps_variant<2>(h, grid, n_steps, w0, w1); break;
+            case 4: launch_steps_variant<4>(h, grid, n_steps, w0, w1); break;
+            case 5: launch_steps_variant<5>(h, grid, n_steps, w0, w1); break;
+            case 6: launch_steps_variant<6>(h, grid, n_steps, w0, w1); break;
+            default: return fail(POS_E_INVALID, "internal error: unknown kernel variant");
+        }
         HIP_TRY(hipGetLastError());
     }
     return POS_OK;
@@ -609,7 +646,7 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
 
     // device order: by GR variant, then body count, then input order
     std::vector<int> variant_in(ns);
-    for (int s = 0; s < ns; ++s) variant_in[s] = gr_variant(d->effects[s], d->gr_implementation[s]);
+    for (int s = 0; s < ns; ++s) variant_in[s] = system_variant(d, s, most_massive(d, s));
     h->sys_perm.resize(ns);
     std::iota(h->sys_perm.begin(), h->sys_perm.end(), 0);
     std::stable_sort(h->sys_perm.begin(), h->sys_perm.end(), [&](int a, int b) {
@@ -634,6 +671,7 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     std::vector<uint32_t> eff(ns), status(ns, 0);
     std::vector<double> dt(ns), tl(ns), t(ns), lh(ns), hp(ns);
     std::vector<long long> iter(ns);
+    std::vector<double> disk(6 * (size_t)ns, 0.);
     for (int ds = 0; ds < ns; ++ds) {
         const int s = h->sys_perm[ds];
         const int b0 = d->body_offset[s], n = d->body_offset[s + 1] - b0;
@@ -645,19 +683,22 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
         coord[ds] = d->coordinates[s];
         uint32_t e = d->effects[s];
         if (d->gr_implementation[s] == POS_GR_DISABLED) e &= ~POS_EFFECT_GENERAL_RELATIVITY;
-        eff[ds] = e | ((uint32_t)h->variant_dev[ds] << EFF_GRV_SHIFT);
+        eff[ds] = e | ((uint32_t)h->variant_dev[ds] << EFF_VAR_SHIFT);
         dt[ds] = d->time_step[s];
         tl[ds] = d->time_limit[s];
         t[ds] = d->current_time ? d->current_time[s] : 0.;
         lh[ds] = d->last_historic_snapshot_time ? d->last_historic_snapshot_time[s] : -1.;
         hp[ds] = d->historic_snapshot_period ? d->historic_snapshot_period[s] : std::numeric_limits<double>::infinity();
         iter[ds] = d->current_iteration ? (long long)d->current_iteration[s] : 0;
+        if (d->disk_properties)
+            for (int q = 0; q < 6; ++q) disk[6 * (size_t)ds + q] = d->disk_properties[6 * (size_t)s + q];
     }
     std::vector<double> x(3 * nb), v(3 * nb), L(3 * nb), sp(3 * nb), ev(3 * nb, 0.), eL(3 * nb, 0.), acc(3 * nb, 0.);
     std::vector<double> m(nb), R(nb), rg2(nb), I(nb), sigma(nb), k2(nb), k2f(nb), wk(nb, 0.), ws(nb, 0.);
+    std::vector<double> sig_scale(nb), sig_factor(nb), zeros_nb(nb, 0.);
     std::vector<uint32_t> roles(nb);
     std::vector<int> evo_type(nb, POS_EVO_NON_EVOLVING), evo_off(nb, 0), evo_len(nb, 0), left(nb, 0);
-    std::vector<double> tab_t, tab_R, tab_rg2, tab_love;
+    std::vector<double> tab_t, tab_R, tab_rg2, tab_love, tab_itq;
     for (int db = 0; db < nb; ++db) {
         const int b = h->body_perm[db];
         m[db] = d->mass[b];
@@ -665,6 +706,8 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
         rg2[db] = d->radius_of_gyration_2[b];
         I[db] = d->moment_of_inertia ? d->moment_of_inertia[b] : m[db] * rg2[db] * (R[db] * R[db]);
         sigma[db] = d->tides_dissipation_factor_scale[b] * d->tides_dissipation_factor[b];
+        sig_scale[db] = d->tides_dissipation_factor_scale[b];
+        sig_factor[db] = d->tides_dissipation_factor[b];
         k2[db] = d->tides_love_number[b];
         k2f[db] = d->flattening_love_number[b];
         if (d->wind_k_factor) wk[db] = d->wind_k_factor[b];
@@ -688,6 +731,10 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
             tab_R.insert(tab_R.end(), d->evolver_radius + o0, d->evolver_radius + o0 + len);
             tab_rg2.insert(tab_rg2.end(), d->evolver_radius_of_gyration_2 + o0, d->evolver_radius_of_gyration_2 + o0 + len);
             tab_love.insert(tab_love.end(), d->evolver_love_number + o0, d->evolver_love_number + o0 + len);
+            if (d->evolver_inverse_tidal_q_factor)
+                tab_itq.insert(tab_itq.end(), d->evolver_inverse_tidal_q_factor + o0, d->evolver_inverse_tidal_q_factor + o0 + len);
+            else
+                tab_itq.insert(tab_itq.end(), (size_t)len, 0.);
         }
     }
 
@@ -700,7 +747,7 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     double *d_dt, *d_tl, *d_t, *d_lh, *d_hp;
     long long *d_iter;
     double *d_x, *d_v, *d_L, *d_sp, *d_ev, *d_eL, *d_acc, *d_R, *d_rg2, *d_I, *d_m, *d_sigma, *d_k2, *d_k2f, *d_wk, *d_ws, *d_roche;
-    double *d_tt, *d_tR, *d_trg2, *d_tlove;
+    double *d_tt, *d_tR, *d_trg2, *d_tlove, *d_titq, *d_sscale, *d_sfactor, *d_lag, *d_mig, *d_disk;
     if ((rc = dalloc(h, &d_bo, ns + 1)) || (rc = dalloc(h, &d_host, ns)) || (rc = dalloc(h, &d_coord, ns)) ||
         (rc = dalloc(h, &d_tw, ns)) || (rc = dalloc(h, &d_eff, ns)) || (rc = dalloc(h, &d_status, ns)) ||
         (rc = dalloc(h, &d_dt, ns)) || (rc = dalloc(h, &d_tl, ns)) || (rc = dalloc(h, &d_t, ns)) ||
@@ -713,7 +760,9 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
         (rc = dalloc(h, &d_ws, nb)) || (rc = dalloc(h, &d_roles, nb)) || (rc = dalloc(h, &d_roche, (size_t)nb * POS_MAX_BODIES)) ||
         (rc = dalloc(h, &d_evo_type, nb)) || (rc = dalloc(h, &d_evo_off, nb)) || (rc = dalloc(h, &d_evo_len, nb)) ||
         (rc = dalloc(h, &d_left, nb)) || (rc = dalloc(h, &d_tt, tab_t.size())) || (rc = dalloc(h, &d_tR, tab_t.size())) ||
-        (rc = dalloc(h, &d_trg2, tab_t.size())) || (rc = dalloc(h, &d_tlove, tab_t.size()))) {
+        (rc = dalloc(h, &d_trg2, tab_t.size())) || (rc = dalloc(h, &d_tlove, tab_t.size())) ||
+        (rc = dalloc(h, &d_titq, tab_t.size())) || (rc = dalloc(h, &d_sscale, nb)) || (rc = dalloc(h, &d_sfactor, nb)) ||
+        (rc = dalloc(h, &d_lag, nb)) || (rc = dalloc(h, &d_mig, nb)) || (rc = dalloc(h, &d_disk, 6 * (size_t)ns))) {
         pos_destroy(h);
         return rc;
     }
@@ -725,7 +774,8 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     E.mass = d_m; E.sigma = d_sigma; E.k2 = d_k2; E.k2f = d_k2f; E.wind_k = d_wk; E.wind_sat = d_ws; E.roles = d_roles;
     E.roche = d_roche;
     E.evo_type = d_evo_type; E.evo_off = d_evo_off; E.evo_len = d_evo_len; E.left_index = d_left;
-    E.tab_time = d_tt; E.tab_radius = d_tR; E.tab_rg2 = d_trg2; E.tab_love = d_tlove;
+    E.tab_time = d_tt; E.tab_radius = d_tR; E.tab_rg2 = d_trg2; E.tab_love = d_tlove; E.tab_itq = d_titq;
+    E.sigma_scale = d_sscale; E.sigma_factor = d_sfactor; E.lag = d_lag; E.disk_migration = d_mig; E.disk_props = d_disk;
 
     if ((rc = upload(h, d_bo, h->body_offset_dev)) || (rc = upload(h, d_host, host_idx)) || (rc = upload(h, d_coord, coord)) ||
         (rc = upload(h, d_tw, tw)) || (rc = upload(h, d_eff, eff)) || (rc = upload(h, d_status, status)) ||
@@ -737,7 +787,9 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
         (rc = upload(h, d_wk, wk)) || (rc = upload(h, d_ws, ws)) || (rc = upload(h, d_roles, roles)) ||
         (rc = upload(h, d_evo_type, evo_type)) || (rc = upload(h, d_evo_off, evo_off)) || (rc = upload(h, d_evo_len, evo_len)) ||
         (rc = upload(h, d_left, left)) || (rc = upload(h, d_tt, tab_t)) || (rc = upload(h, d_tR, tab_R)) ||
-        (rc = upload(h, d_trg2, tab_rg2)) || (rc = upload(h, d_tlove, tab_love))) {
+        (rc = upload(h, d_trg2, tab_rg2)) || (rc = upload(h, d_tlove, tab_love)) || (rc = upload(h, d_titq, tab_itq)) ||
+        (rc = upload(h, d_sscale, sig_scale)) || (rc = upload(h, d_sfactor, sig_factor)) ||
+        (rc = upload(h, d_lag, zeros_nb)) || (rc = upload(h, d_mig, zeros_nb)) || (rc = upload(h, d_disk, disk))) {
         pos_destroy(h);
         return rc;
     }
@@ -1044,15 +1096,19 @@ int32_t pos_additional_effects(pos_ens *h, double *out_acc, double *out_dLdt) {
                        (rc = dalloc(h, &h->d_op_st, h->n_systems))))
         return rc;
     double *d_a = h->d_op_a, *d_l = h->d_op_l;
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < N_VARIANTS; ++v) {
         const int w0 = h->variant_waves[v][0], w1 = h->variant_waves[v][1];
         if (w1 <= w0) continue;
-        if (v == GRV_KIDDER)
-            hipLaunchKernelGGL(effects_kernel<GRV_KIDDER>, dim3(w1 - w0), dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1);
-        else if (v == GRV_ANDERSON)
-            hipLaunchKernelGGL(effects_kernel<GRV_ANDERSON>, dim3(w1 - w0), dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1);
-        else
-            hipLaunchKernelGGL(effects_kernel<GRV_NEWHALL>, dim3(w1 - w0), dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1);
+        const dim3 g(w1 - w0);
+        switch (v) {
+            case 0: hipLaunchKernelGGL(effects_kernel<0>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 1: hipLaunchKernelGGL(effects_kernel<1>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 2: hipLaunchKernelGGL(effects_kernel<2>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 4: hipLaunchKernelGGL(effects_kernel<4>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 5: hipLaunchKernelGGL(effects_kernel<5>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 6: hipLaunchKernelGGL(effects_kernel<6>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            default: return fail(POS_E_INVALID, "internal error: unknown kernel variant");
+        }
         HIP_TRY(hipGetLastError());
     }
     std::vector<double> a, l;
@@ -1221,9 +1277,9 @@ int32_t pos_history_drain(pos_ens *h, pos_history_record *out, int64_t max_recor
                 o.radius_of_gyration_2 = f(HF_RG2);
                 o.love_number = f(HF_LOVE);
                 o.scaled_dissipation_factor = h->sigma_dev[db];
-                o.lag_angle = 0.;
+                o.lag_angle = f(HF_LAG);
                 o.denergy_dt = f(HF_DEDT);
-                o.migration_timescale = 0.;
+                o.migration_timescale = f(HF_MIGRATION);
             }
         }
     }
@@ -1244,8 +1300,10 @@ int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
             return fail(POS_E_UNSUPPORTED, where + "the adjoint supports democratic-heliocentric coordinates only");
         if (h->eff_dev[ds] & (POS_EFFECT_WIND | POS_EFFECT_DISK | POS_EFFECT_EVOLUTION))
             return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support wind, disk or evolution");
-        if (h->variant_dev[ds] != GRV_KIDDER)
+        if ((h->variant_dev[ds] & 3) != GRV_KIDDER)
             return fail(POS_E_UNSUPPORTED, where + "the adjoint supports Kidder1995 general relativity only");
+        if (h->variant_dev[ds] & VAR_EXT)
+            return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support evolving dissipation");
         for (int b = h->body_offset_dev[ds]; b < h->body_offset_dev[ds + 1]; ++b)
             if (h->evo_dev[b] != POS_EVO_NON_EVOLVING)
                 return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support evolving bodies");

@@ -30,6 +30,7 @@
 #include <stdint.h>
 
 #include "../../include/posidonius_hip.h"
+#include "crmath.hpp"
 
 namespace posdev {
 
@@ -47,6 +48,10 @@ constexpr double R_SUN = 6.957e8 / AU;
 constexpr double G = G_SI / (AU * AU * AU) * M_SUN * (DAY * DAY);
 constexpr double K2 = G;
 constexpr double MAX_DISTANCE_2 = 100. * 100.;
+constexpr double SUN_DYN_FREQ_2 = K2 / (R_SUN * R_SUN * R_SUN);
+constexpr double SMOOTHING_FACTOR_DYN_TIDE_COROTATION = 1.0e-5 * 86400.;
+constexpr double BOLTZMANN_CONSTANT = 1.380649e-23 / (M_SUN * AU * AU) * DAY * DAY;
+constexpr double MASS_HYDROGEN_ATOM = 1.672162841e-27 / M_SUN;
 constexpr int NMAX_QUART = 64;
 constexpr int NMAX_NEWT = 32;
 constexpr int MIDPOINT_MIN_ITER = 3;
@@ -98,12 +103,18 @@ struct DevEnsemble {
     double *x, *v, *L, *spin, *err_v, *err_L, *acc;   // [3][n_bodies] component-major
     double *radius, *rg2, *inertia;                    // [n_bodies]
     const double *mass, *sigma, *k2, *k2f, *wind_k, *wind_sat;
+    const double *sigma_scale, *sigma_factor;          // the two factors of sigma
     const uint32_t *roles;
     const double *roche;                               // [n_bodies][POS_MAX_BODIES]
     // evolution
     const int *evo_type, *evo_off, *evo_len;
     int *left_index;
-    const double *tab_time, *tab_radius, *tab_rg2, *tab_love;
+    const double *tab_time, *tab_radius, *tab_rg2, *tab_love, *tab_itq;
+    // evolving dissipation and disk: lag angle of each body (evolution.rs:548-567),
+    // migration timescale of its last disk evaluation (disk.rs:236), and the
+    // disk host's DiskProperties per system (6 doubles)
+    double *lag, *disk_migration;
+    const double *disk_props;
     // launch table: wave w covers device systems wave_first[w] .. +wave_count[w]-1
     const int *wave_first, *wave_count, *wave_group;
     int n_waves;
@@ -119,8 +130,14 @@ struct DevEnsemble {
 // migration timescale, src/integrator/output.rs:90-134).
 enum HistField {
     HF_TIME = 0, HF_X = 1, HF_SPIN = 4, HF_V = 7, HF_RADIUS = 10, HF_RG2 = 11, HF_LOVE = 12, HF_DEDT = 13,
-    HIST_FIELDS = 14
+    HF_LAG = 14, HF_MIGRATION = 15, HIST_FIELDS = 16
 };
+
+// evolving-dissipation models (constant_time_lag.rs:26, 100)
+__host__ __device__ __forceinline__ bool evo_dynamical(int type) {
+    return type == POS_EVO_BOLMONT_MATHIS2016 || type == POS_EVO_GALLET_BOLMONT2017 ||
+           type == POS_EVO_LECONTE_CHABRIER2013_DYN;
+}
 
 // ---- LDS of one wavefront ---------------------------------------------------
 constexpr int NX = 24;   // exchange slots per lane
@@ -139,7 +156,11 @@ __device__ __forceinline__ void wsync() { __syncthreads(); }
 // ---- per-lane registers -----------------------------------------------------
 enum : uint32_t {
     F_VALID = 1u << 0, F_HOST = 1u << 1, F_TIDES = 1u << 2, F_RF = 1u << 3, F_GR = 1u << 4,
-    F_WIND = 1u << 5, F_EVOLUTION = 1u << 6, F_SPIN = 1u << 7, F_WIND_ROLE = 1u << 8, F_LC_LOVE = 1u << 9
+    F_WIND = 1u << 5, F_EVOLUTION = 1u << 6, F_SPIN = 1u << 7, F_WIND_ROLE = 1u << 8, F_LC_LOVE = 1u << 9,
+    F_DISK = 1u << 10,        // the system has the disk effect
+    F_DISK_ROLE = 1u << 11,   // this body is a disk orbiting body
+    F_DYN_HOST = 1u << 12,    // the host has an evolving-dissipation model
+    F_DYN = 1u << 13          // this body has an evolving-dissipation model
 };
 struct Lane {
     int lane, gbase, k, n, host, coord, sys, body;
@@ -161,6 +182,7 @@ struct Lane {
     __device__ __forceinline__ bool rf() const { return flags & F_RF; }
     __device__ __forceinline__ bool gr() const { return flags & F_GR; }
     __device__ __forceinline__ bool spin_on() const { return flags & F_SPIN; }
+    __device__ __forceinline__ int host_body() const { return body - k + host; }
     __device__ __forceinline__ int hl() const { return gbase + host; }
 };
 
@@ -233,6 +255,20 @@ __device__ __forceinline__ void own_spin(WaveLds &lds, Lane &B) {
     lds.s2[B.lane] = p2(s0) + p2(s1) + p2(s2);
 }
 
+// evolution.rs:548-567 for this lane's body (after its spin is refreshed):
+// the lag angle of an evolving-dissipation model, from the inverse tidal Q
+// factor interpolated at t (which moves the evolver's left_index too).
+__device__ inline void evolve_spin_dependent(const DevEnsemble &E, WaveLds &lds, Lane &B, double t) {
+    if (!B.valid() || !(B.flags & F_DYN)) return;
+    const int off = E.evo_off[B.body], len = E.evo_len[B.body];
+    const int i0 = evolver_idx(B.left_index, len);
+    int left = 0;
+    const double itq = interp_table(E.tab_time + off + i0, E.tab_itq + off + i0, len - i0, t, &left);
+    B.left_index += left;
+    const double epsilon_squared = lds.s2[B.lane] / SUN_DYN_FREQ_2;
+    E.lag[B.body] = 3.0 * epsilon_squared * itq / 4.0;
+}
+
 // ===========================================================================
 // Additional effects + implicit midpoint (whfast.rs:322-466, universe.rs:353-614)
 // ===========================================================================
@@ -276,9 +312,14 @@ __device__ __forceinline__ void host_reaction_sums(WaveLds &lds, const Lane &B) 
 // universe.rs:540-566) and use the exchange rows freely.
 // ===========================================================================
 enum GrVariant { GRV_KIDDER = 0, GRV_ANDERSON = 1, GRV_NEWHALL = 2 };
-// The device copy of a system's effect mask carries its variant in these bits.
-constexpr uint32_t EFF_GRV_SHIFT = 16;
-__host__ __device__ __forceinline__ int eff_gr_variant(uint32_t eff) { return (int)((eff >> EFF_GRV_SHIFT) & 3u); }
+// Kernel variant: the GR variant plus VAR_EXT for systems with the disk effect
+// or an evolving-dissipation host (the extended effects pass).  The device
+// copy of a system's effect mask carries its variant in bits 16-18.
+constexpr int VAR_EXT = 4;
+constexpr int N_VARIANTS = 8;
+constexpr uint32_t EFF_VAR_SHIFT = 16;
+__host__ __device__ __forceinline__ int eff_variant(uint32_t eff) { return (int)((eff >> EFF_VAR_SHIFT) & 7u); }
+__host__ __device__ __forceinline__ int eff_gr_variant(uint32_t eff) { return eff_variant(eff) & 3; }
 
 // Chained order of general_relativity.rs (host, then the other bodies by
 // index): position of body k, and body at position c.
@@ -533,13 +574,62 @@ __device__ inline void gr_newhall1983(WaveLds &lds, Lane &B, double g[3]) {
     g[0] = anew[0]; g[1] = anew[1]; g[2] = anew[2];
 }
 
+// tools.rs:251-285 (Mercury's perihelion distance and eccentricity)
+__device__ inline void perihelion_distance_and_eccentricity(double gm, const double x[3], const double u[3], double &q,
+                                                            double &e) {
+    const double hx = x[1] * u[2] - x[2] * u[1];
+    const double hy = x[2] * u[0] - x[0] * u[2];
+    const double hz = x[0] * u[1] - x[1] * u[0];
+    const double h2 = hx * hx + hy * hy + hz * hz;
+    const double v2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const double sv = h2 / gm;
+    const double temp = 1. + sv * (v2 / gm - 2. / r);
+    e = temp <= 0. ? 0. : sqrt(temp);
+    q = sv / (1. + e);
+}
+
+// Disk force on an orbiting body (disk.rs:155-268, Alibert et al. 2013) from
+// its heliocentric position and velocity; stores its migration timescale.
+__device__ inline void disk_force(const DevEnsemble &E, const WaveLds &lds, const Lane &B, const double px[3],
+                                  const double pv[3], double d, double nv2, double F[3]) {
+    const double *dp = E.disk_props + 6 * B.sys;   // inner, outer, lifetime, alpha, sigma0, mu
+    const int hl = B.hl();
+    const double hm = lds.m[hl], hmg = lds.mg[hl], m = B.m, mg = lds.mg[B.lane];
+    const double sma = (mg + hmg) * d / (2.0 * (mg + hmg) - d * nv2);
+    // gas radial velocity from the alpha viscosity (disk.rs:206-226)
+    const double keplerian_frequency = sqrt(hmg / (d * (d * d)));
+    const double temperature = 280.0 * poscr::cr_rsqrt(d) * hm;
+    const double speed_of_sound_2 = BOLTZMANN_CONSTANT * temperature / (dp[5] * MASS_HYDROGEN_ATOM);
+    const double viscosity = dp[3] * speed_of_sound_2 / keplerian_frequency;
+    const double gas_radial_velocity = -3.0 * viscosity / (2.0 * sma);
+    // surface density (disk.rs:196-205)
+    const double initial = dp[4] * (1. / d) * poscr::cr_exp(-1.0 * d / dp[1]) * (1.0 - sqrt(dp[0] / d));
+    double surface_density = 0.0;
+    if (d > dp[0]) surface_density = initial * poscr::cr_exp(-B.t / dp[2]);
+    const double ratio = 2.0 * surface_density * p2(sma) / m;
+    const double a_dot = gas_radial_velocity * ((ratio != ratio || ratio > 1.0) ? 1.0 : ratio);   // f64::min(1, .)
+    const double timescale = -sma / a_dot;
+    E.disk_migration[B.body] = timescale;
+    const double factor_migration = -m / timescale;
+    const double eccentricity_timescale = 0.1 * timescale;
+    const double factor_eccentricity = -m * 2.0 / eccentricity_timescale;
+    const double factor_inclination = -m * 2.0 / eccentricity_timescale;
+    const double sp = 1.0 / (d * d) * (px[0] * pv[0] + px[1] * pv[1] + px[2] * pv[2]);
+    F[0] = factor_migration * pv[0] + factor_eccentricity * sp * px[0] + 0.0;
+    F[1] = factor_migration * pv[1] + factor_eccentricity * sp * px[1] + 0.0;
+    F[2] = factor_migration * pv[2] + factor_eccentricity * sp * px[2] + factor_inclination * pv[2];
+}
+
 // Accelerations and spin derivatives of every body of the group at the current
 // (averaged) state.  Outputs are accumulated in the reference's order:
 // a_add = ((0 + tides) + flattening) + GR and
 // dL/dt = ((tides + flattening) + GR) + wind (universe.rs:540-614).
-template <int GRV>
+template <int V>
 __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, Lane &B, bool evolution,
                                           double a_add[3], double dLdt[3]) {
+    constexpr int GRV = V & 3;
+    constexpr bool EXT = (V & VAR_EXT) != 0;
     const int lane = B.lane;
     const int hl = B.hl();
     const bool orbiting = B.orbiting();
@@ -575,7 +665,39 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
     // ---- evolution and spin = L / I (universe.rs:305-316) ----
     if (evolution && (B.flags & F_EVOLUTION)) evolve_body(E, lds, B, B.t);
     own_spin(lds, B);
+    double dk[3] = {0., 0., 0.};   // disk acceleration of this body
+    double host_lag = 0.;
+    if (EXT) {
+        if (evolution && (B.flags & F_EVOLUTION)) evolve_spin_dependent(E, lds, B, B.t);
+        // disk forces of the orbiting bodies (rows 0-2, row 3 = 1 for a disk
+        // body) and the host's lag angle (row 3 of its column) for the host
+        double F[3] = {0., 0., 0.};
+        if (orbiting && (B.flags & F_DISK) && (B.flags & F_DISK_ROLE)) {
+            disk_force(E, lds, B, px, pv, d, nv2, F);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) dk[q] = B.inv_m * F[q];
+        }
+        if (B.valid()) {
+            lds.xb[0][lane] = F[0]; lds.xb[1][lane] = F[1]; lds.xb[2][lane] = F[2];
+            lds.xb[3][lane] = B.is_host() ? E.lag[B.body] : ((B.flags & F_DISK_ROLE) ? 1. : 0.);
+        }
+    }
     wsync();
+    if (EXT) {
+        if (B.valid() && B.is_host() && (B.flags & F_DISK)) {   // disk.rs:259-265, bodies in index order
+            double sum[3] = {0., 0., 0.};
+            for (int j = 0; j < B.n; ++j) {
+                const int l = B.gbase + j;
+                if (j == B.host || lds.xb[3][l] == 0.) continue;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) sum[q] = sum[q] + lds.xb[q][l];
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) dk[q] = -1.0 * B.inv_m * sum[q];
+        }
+        host_lag = lds.xb[3][hl];
+        wsync();
+    }
 
 #pragma unroll
     for (int q = 0; q < 3; ++q) { a_add[q] = 0.; dLdt[q] = 0.; }
@@ -589,10 +711,28 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         const double d7 = p7(d);
         const Rcp Rd7 = rcp(d7);
         const double d5 = p5(d);
+        // ---- the host's dissipation factor in this pair: its own, or with an
+        //      evolving-dissipation host whose dynamical tide this body
+        //      excites, the pair value (constant_time_lag.rs:20-94) ----
+        double sigma_host = lds.sigma[hl];
+        if (EXT && (B.flags & F_DYN_HOST) && B.tides()) {
+            const double host_norm_spin = sqrt(hs2);
+            const double gm = lds.mg[hl] + lds.mg[lane];
+            double q, e;
+            perihelion_distance_and_eccentricity(gm, px, pv, q, e);
+            const double mean_motion = sqrt(gm) * poscr::cr_rsqrt3(q / (1.0 - e));
+            double half = fabs(host_norm_spin - mean_motion);
+            if (half < host_norm_spin) {
+                if (half < SMOOTHING_FACTOR_DYN_TIDE_COROTATION) half = SMOOTHING_FACTOR_DYN_TIDE_COROTATION;
+                const double inverse_of_half = 1. / half;
+                const int hb = B.host_body();
+                sigma_host = E.sigma_scale[hb] * (2.0 * K2 / (3.0 * p5(hR)) * host_lag * inverse_of_half + E.sigma_factor[hb]);
+            }
+        }
         // ---- tides: orthogonal components (constant_time_lag.rs:206-264) ----
         double os = 0., op = 0.;
         if (B.tides()) {
-            os = divr(4.5 * p2(m) * p10(hR) * lds.sigma[hl], Rd7);
+            os = divr(4.5 * p2(m) * p10(hR) * sigma_host, Rd7);
             op = divr(4.5 * p2(hm) * p10(R) * lds.sigma[lane], Rd7);
         }
         // ---- flattening: orthogonal components (oblate_spheroid.rs:21-49) ----
@@ -609,7 +749,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
             const double pm2 = m * m;
             const double conservative = divr(-3.0 * K2, Rd7) * (pm2 * p5(hR) * lds.k2[hl] + hm2 * p5(R) * lds.k2[lane]);
             const double factor1 = -13.5 * vr / p8(d);
-            const double term1 = pm2 * p10(hR) * lds.sigma[hl];
+            const double term1 = pm2 * p10(hR) * sigma_host;
             const double term2 = hm2 * p10(R) * lds.sigma[lane];
             const double diss_pm = factor1 * term2;
             const double dissipative = diss_pm + factor1 * term1;
@@ -627,6 +767,10 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
             a_add[0] = a_add[0] + B.inv_m * F0;
             a_add[1] = a_add[1] + B.inv_m * F1;
             a_add[2] = a_add[2] + B.inv_m * F2;
+        }
+        if (EXT && (B.flags & F_DISK)) {   // universe.rs:550-554: after tides, before flattening
+#pragma unroll
+            for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + dk[q];
         }
         if (B.rf()) {
             // radial component (oblate_spheroid.rs:51-60) and force (83-97)
@@ -780,6 +924,10 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
                 for (int q = 0; q < 3; ++q) dLdt[q] = row(C_TN + q);
             }
         }
+        if (EXT && (B.flags & F_DISK)) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + dk[q];
+        }
         if (B.rf()) {      // rotational_flattening/common.rs:203-237 and 165-201
 #pragma unroll
             for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + -1.0 * B.inv_m * row(C_RF + q);
@@ -830,7 +978,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
     for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + w[q];
 }
 
-template <int GRV>
+template <int V>
 __device__ inline void integrate_velocity_dependent_forces(const DevEnsemble &E, WaveLds &lds, Lane &B, bool evolution) {
     const double h = 0.5 * B.dt;
     const bool spin_on = B.spin_on();
@@ -840,7 +988,7 @@ __device__ inline void integrate_velocity_dependent_forces(const DevEnsemble &E,
     bool converged = false;
     for (int it = 0; it < MIDPOINT_MAX_ITER; ++it) {
         double a_add[3], dLdt[3];
-        additional_effects<GRV>(E, lds, B, evolution && it == 0, a_add, dLdt);
+        additional_effects<V>(E, lds, B, evolution && it == 0, a_add, dLdt);
         double pdv = 0., pdL = 0.;   // |change of the final state|^2 of this body
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -1410,6 +1558,7 @@ __device__ inline void merge_status(WaveLds &lds, Lane &B) {
 // state: the reference evaluates it from the tidal internals of the last
 // effects evaluation of the previous step, half a step earlier.
 // ===========================================================================
+template <bool EXT>
 __device__ inline void write_history_record(const DevEnsemble &E, WaveLds &lds, Lane &B) {
     const int lane = B.lane;
     if (B.valid()) {
@@ -1462,6 +1611,8 @@ __device__ inline void write_history_record(const DevEnsemble &E, WaveLds &lds, 
         rec[HF_RG2 * nb] = E.rg2[B.body];
         rec[HF_LOVE * nb] = lds.k2[lane];
         rec[HF_DEDT * nb] = dedt;
+        rec[HF_LAG * nb] = EXT ? E.lag[B.body] : 0.;
+        rec[HF_MIGRATION * nb] = EXT ? E.disk_migration[B.body] : 0.;
     }
     B.hcount = slot + 1;
 }
@@ -1494,7 +1645,7 @@ __device__ __forceinline__ void tape_put3(const TapeRef &T, int row, int field, 
 // WHFast::iterate (whfast.rs:235-305).  With REC, the step also writes its
 // tape row `row` (and the end state into row + 1).
 // ===========================================================================
-template <bool REC, int GRV>
+template <bool REC, int V>
 __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, const TapeRef &T, int row) {
     {
         const bool first = B.last_hist < 0.;
@@ -1502,8 +1653,9 @@ __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, cons
         if (first || B.last_hist + period <= B.t) {
             if (B.flags & F_EVOLUTION) evolve_body(E, lds, B, B.t);
             own_spin(lds, B);
+            if ((V & VAR_EXT) && (B.flags & F_EVOLUTION)) evolve_spin_dependent(E, lds, B, B.t);
             wsync();
-            if (E.hist) write_history_record(E, lds, B);
+            if (E.hist) write_history_record<(V & VAR_EXT) != 0>(E, lds, B);
             B.last_hist = first ? 0. : B.last_hist + period;
         }
     }
@@ -1513,7 +1665,7 @@ __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, cons
         tape_put3(T, row, TP_V, B.body, B.v);
         tape_put3(T, row, TP_L, B.body, B.L);
     }
-    integrate_velocity_dependent_forces<GRV>(E, lds, B, true);
+    integrate_velocity_dependent_forces<V>(E, lds, B, true);
     if (REC && B.valid()) {
         tape_put3(T, row, TP_V1, B.body, B.v);
         tape_put3(T, row, TP_L1, B.body, B.L);
@@ -1529,7 +1681,7 @@ __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, cons
     kepler_steps(lds, B, half_dt);
     if (REC && B.valid()) *T.at(row, TP_XK2, B.body) = B.orbiting() ? B.kx : 0.;
     alternative_to_inertial(lds, B);
-    integrate_velocity_dependent_forces<GRV>(E, lds, B, false);
+    integrate_velocity_dependent_forces<V>(E, lds, B, false);
     if (REC && B.valid()) {
         tape_put3(T, row + 1, TP_X, B.body, B.x);
         tape_put3(T, row + 1, TP_V, B.body, B.v);

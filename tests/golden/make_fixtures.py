@@ -26,7 +26,7 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 
 PARTICLE_KEYS = ("mass", "radius", "radius_of_gyration_2", "moment_of_inertia", "inertial_position",
                  "inertial_velocity", "spin", "angular_momentum", "heliocentric_velocity", "evolution")
-EFFECT_KEYS = ("tides", "rotational_flattening", "general_relativity")
+EFFECT_KEYS = ("tides", "rotational_flattening", "general_relativity", "disk")
 WINDOW = 12
 
 

@@ -70,5 +70,20 @@ def test_invalid_desc_is_rejected_before_any_device_work(lib):
     h = spec.desc()
     out = C.c_void_p()
     rc = lib.pos_create(h.ref, None, C.byref(out))
-    assert rc == abi.E_UNSUPPORTED
-    assert b"disk" in lib.pos_last_error()
+    assert rc == abi.E_INVALID
+    assert b"disk_properties" in lib.pos_last_error()
+    # properties given, but the host is no disk central body
+    spec.system["disk_properties"] = np.tile([0.01, 100.0, 3.6525e7, 0.01, 1e-4, 2.4], (spec.n_systems, 1))
+    h = spec.desc()
+    assert lib.pos_create(h.ref, None, C.byref(out)) == abi.E_UNSUPPORTED
+    assert b"disk host" in lib.pos_last_error()
+
+
+def test_dynamical_tides_of_an_orbiting_body_are_rejected(lib):
+    from tests.oracle_runner import load_golden
+    spec, _ = load_golden("test_evolution-solar_like_bolmontmathis2016")
+    spec.body["evolution_type"][2] = abi.EVO_LECONTE_CHABRIER2013_DYN
+    h = spec.desc()
+    out = C.c_void_p()
+    assert lib.pos_create(h.ref, None, C.byref(out)) == abi.E_UNSUPPORTED
+    assert b"orbiting body" in lib.pos_last_error()

@@ -16,9 +16,7 @@ from tests.oracle_runner import golden_names, load_golden
 
 pytestmark = pytest.mark.gpu
 
-UNSUPPORTED = {"test_disk-enabled_disk", "test_evolution-solar_like_bolmontmathis2016",
-               "test_evolution-solar_like_galletbolmont2017"}
-SUPPORTED = [n for n in golden_names() if n not in UNSUPPORTED]
+SUPPORTED = golden_names()
 STATE_KEYS = ("position", "velocity", "acceleration", "angular_momentum", "spin", "velocity_error",
               "angular_momentum_error", "radius", "moment_of_inertia", "current_time", "current_iteration",
               "status", "evolver_left_index", "timestep_warning")
@@ -47,7 +45,7 @@ def assert_same_state(gpu, ref, keys=STATE_KEYS, where=""):
 
 
 def test_fixtures_one_ensemble_bitwise(Ensemble):
-    """All 23 supported reference fixtures in one packed ensemble, run to
+    """All 26 reference fixtures in one packed ensemble, run to
     completion (199 steps): positions, velocities and accelerations equal the
     reference's particle_<i>.json bit for bit."""
     specs, expected = [], []
@@ -226,3 +224,90 @@ def test_adjoint_rejects_other_gr_variants(Ensemble):
     with Ensemble(spec) as ens:
         with pytest.raises(Exception, match="Kidder1995"):
             ens.adjoint_record(10, 5)
+
+
+def excited_dynamical_tide():
+    """The BolmontMathis2016 fixture with a hot-Jupiter-mass first planet and
+    the inverse tidal Q factor raised 1e4 times: the star's dynamical tide is
+    excited (constant_time_lag.rs:57-64) and the pair factor moves the state
+    by ~2e-14 AU within 199 steps."""
+    spec, _ = load_golden("test_evolution-solar_like_bolmontmathis2016")
+    s = spec.copy()
+    for k in ("mass", "moment_of_inertia"):
+        s.body[k][1] *= 300.0
+    s.body["angular_momentum"][1] *= 300.0
+    s.tables["evolver_inverse_tidal_q_factor"] = s.tables["evolver_inverse_tidal_q_factor"] * 1e4
+    return s
+
+
+def extended_ensemble():
+    """Systems of the extended effects pass (the disk and evolving dissipation)
+    among plain ones: the three reference fixtures, the disk fixture with its
+    surface density lowered 1e7 times (below the min(1, 2 Sigma a^2 / m)
+    saturation, so the surface density and both exponentials enter), the
+    excited dynamical tide, and uneven / Jacobi systems; shuffled."""
+    parts = [load_golden(n)[0] for n in ("test_disk-enabled_disk", "test_evolution-solar_like_bolmontmathis2016",
+                                         "test_evolution-solar_like_galletbolmont2017")]
+    thin = parts[0].copy()
+    thin.system["disk_properties"][0, 4] *= 1e-7
+    parts += [thin, excited_dynamical_tide(), gen.config2(16, seed=31), gen.config3(8, seed=32)]
+    spec = EnsembleSpec.concat(parts)
+    return spec.subset(np.random.default_rng(9).permutation(spec.n_systems))
+
+
+@pytest.mark.parametrize("mapping,group_size,graph", [("packed", -1, True), ("packed", 0, False), ("plain", 0, False)])
+def test_extended_effects_against_oracle(Ensemble, oracle, mapping, group_size, graph):
+    """Disk forces (disk.rs:155-268), evolving-dissipation lag angles and pair
+    factors (evolution.rs:548-567, constant_time_lag.rs:20-94) in their own
+    kernel variant: bitwise equal to the oracle."""
+    spec = extended_ensemble()
+    steps = 150
+    ref = oracle.run(spec, steps, threads=8)
+    log(f"extended effects mapping={mapping} G={group_size} graph={graph}: {spec.n_systems} systems x {steps} steps")
+    with Ensemble(spec, mapping=mapping, group_size=group_size, steps_per_launch=16, use_graph=graph) as ens:
+        ens.step(steps)
+        out = ens.get_state()
+    assert_same_state(out, ref, where=f"extended {mapping} G={group_size}")
+
+
+def test_extended_effects_operators_against_oracle(Ensemble, oracle):
+    spec = extended_ensemble()
+    with Ensemble(spec) as ens:
+        e_acc, e_dl = ens.additional_effects()
+    r_eacc, r_edl = oracle.additional_effects(spec)
+    np.testing.assert_array_equal(e_acc, r_eacc)
+    np.testing.assert_array_equal(e_dl, r_edl)
+
+
+def test_history_carries_lag_angle_and_migration_timescale(Ensemble):
+    """output.rs:131-134: the lag angle of the evolving-dissipation star
+    (3 |Omega|^2 / SUN_DYN_FREQ_2 * Q^-1(t) / 4, evolution.rs:553-560) and the
+    disk migration timescale of the disk's planets."""
+    names = ("test_evolution-solar_like_bolmontmathis2016", "test_disk-enabled_disk")
+    spec = EnsembleSpec.concat([load_golden(n)[0] for n in names])
+    spec.system["historic_snapshot_period"][:] = 10 * spec.system["time_step"]
+    with Ensemble(spec, steps_per_launch=8) as ens:
+        ens.enable_history(32)
+        ens.step(60)
+        rec = ens.drain_history()
+    bm = rec[rec["system"] == 0]
+    star = bm[bm["body"] == 0]
+    assert len(star) == 6 and np.all(star["lag_angle"] > 0)
+    assert np.all(bm["lag_angle"][bm["body"] > 0] == 0)
+    K2 = (6.67428e-11 / 1.49597870700e11 ** 3) * 1.9818e30 * 86400.0 ** 2
+    sun_dyn_freq_2 = K2 / (6.957e8 / 1.49597870700e11) ** 3
+    b0 = spec.evolver_offset[0]
+    n0 = spec.evolver_offset[1] - b0
+    t_tab = spec.tables["evolver_time"][b0:b0 + n0]
+    q_tab = spec.tables["evolver_inverse_tidal_q_factor"][b0:b0 + n0]
+    omega2 = (star["spin"] ** 2).sum(axis=1)
+    expected = 3.0 * omega2 / sun_dyn_freq_2 * np.interp(star["time"], t_tab, q_tab) / 4.0
+    np.testing.assert_allclose(star["lag_angle"], expected, rtol=1e-12)
+    disk = rec[rec["system"] == 1]
+    planets = disk[disk["body"] > 0]
+    first = planets["time"] == 0.0   # before the first evaluation
+    assert np.all(planets["migration_timescale"][first] == 0)
+    assert np.all(np.isfinite(planets["migration_timescale"][~first]))
+    assert np.all(planets["migration_timescale"][~first] > 0)
+    assert np.all(disk["migration_timescale"][disk["body"] == 0] == 0)
+    assert np.all(bm["migration_timescale"] == 0)

@@ -13,18 +13,12 @@ import pytest
 from posidonius_amd import abi
 from tests.oracle_runner import golden_names, load_golden
 
-UNSUPPORTED = {
-    "test_disk-enabled_disk": "disk effect",
-    "test_evolution-solar_like_bolmontmathis2016": "evolving dissipation",
-    "test_evolution-solar_like_galletbolmont2017": "evolving dissipation",
-}
-SUPPORTED = [n for n in golden_names() if n not in UNSUPPORTED]
+SUPPORTED = golden_names()
 
 
 def test_fixture_inventory():
     # 26 WHFast fixture sets in the reference (IAS15/LeapFrog excluded)
     assert len(golden_names()) == 26
-    assert len(SUPPORTED) == 23
 
 
 @pytest.mark.parametrize("name", SUPPORTED)
@@ -40,11 +34,39 @@ def test_oracle_matches_reference_fixture(oracle, name):
         np.testing.assert_array_equal(out[key], expected[key])  # and in fact bit for bit
 
 
-@pytest.mark.parametrize("name", sorted(UNSUPPORTED))
-def test_oracle_rejects_unsupported_fixture(oracle, name):
-    spec, _ = load_golden(name)
-    with pytest.raises(RuntimeError, match=UNSUPPORTED[name]):
-        oracle.run(spec, n_steps=1)
+def test_disk_fixture_depends_on_the_disk_forces(oracle):
+    """The disk fixture pins the migration and damping forces (disk.rs:228-268):
+    without them the state moves by ~3e-6 AU.  It does not pin the surface
+    density (disk.rs:196-205): min(1, 2 Sigma a^2 / m) saturates at 1 for its
+    planets, so a change of Sigma leaves every bit in place."""
+    spec, expected = load_golden("test_disk-enabled_disk")
+    off = spec.copy()
+    off.system["effects"][:] &= ~np.uint32(abi.EFFECT_DISK)
+    assert np.abs(oracle.run(off, 10**9)["position"] - expected["position"]).max() > 1e-7
+    denser = spec.copy()
+    denser.system["disk_properties"][0, 4] *= 1.5
+    np.testing.assert_array_equal(oracle.run(denser, 10**9)["position"], expected["position"])
+
+
+def test_evolving_dissipation_fixtures_stay_in_the_equilibrium_regime(oracle):
+    """In the BolmontMathis2016 / GalletBolmont2017 fixtures every planet's
+    mean motion exceeds twice the star's spin, so the dynamical tide is never
+    excited (constant_time_lag.rs:57-64) and the pair factor equals the star's
+    own: a non-evolving-dissipation star gives the same bits.  The excited
+    branch is therefore not pinned by a reference vector."""
+    for name in ("test_evolution-solar_like_bolmontmathis2016", "test_evolution-solar_like_galletbolmont2017"):
+        spec, expected = load_golden(name)
+        plain = spec.copy()
+        plain.body["evolution_type"][0] = abi.EVO_BARAFFE1998
+        np.testing.assert_array_equal(oracle.run(plain, 10**9)["position"], expected["position"])
+
+
+def test_oracle_rejects_dynamical_tides_of_an_orbiting_body(oracle):
+    spec, _ = load_golden("test_evolution-solar_like_bolmontmathis2016")
+    bad = spec.copy()
+    bad.body["evolution_type"][2] = abi.EVO_LECONTE_CHABRIER2013_DYN
+    with pytest.raises(RuntimeError, match="evolving dissipation of an orbiting body"):
+        oracle.run(bad, n_steps=1)
 
 
 def _swap_first_two(spec):
