@@ -121,7 +121,8 @@ struct Case {
     std::vector<double> mass, radius, rg2, inertia, position, velocity, spin, angular_momentum, helio_velocity;
     std::vector<double> sigma, sigma_scale, k2, k2f, wind_k, wind_sat, err_v, err_L;
     std::vector<int32_t> evo_type, evo_offset, evo_left, ids;
-    std::vector<double> tab_time, tab_radius, tab_rg2, tab_love;
+    std::vector<double> tab_time, tab_radius, tab_rg2, tab_love, tab_itq;
+    std::vector<double> disk_properties = std::vector<double>(6, 0.);   // the disk host's DiskProperties
 
     explicit Case(Value d) : doc(std::move(d)) {
         if (!doc.find("alternative_coordinates_type")) throw std::runtime_error("not a WHFast case");
@@ -187,6 +188,18 @@ struct Case {
             tag = enum_tag(p.at("general_relativity").at("effect"), &pl);
             if (tag == "CentralBody" && !(pl && pl->is_string() && pl->str() == "Disabled")) r |= POS_ROLE_GR_CENTRAL;
             else if (tag == "OrbitingBody") r |= POS_ROLE_GR_ORBITING;
+            if (const Value *disk = p.find("disk")) {   // disk.rs:44-49
+                const Value *props;
+                tag = enum_tag(disk->at("effect"), &props);
+                if (tag == "CentralBody") {
+                    r |= POS_ROLE_DISK_CENTRAL;
+                    const char *keys[6] = {"inner_edge_distance", "outer_edge_distance", "lifetime", "alpha",
+                                           "surface_density_normalization", "mean_molecular_weight"};
+                    for (int q = 0; q < 6; ++q) disk_properties[q] = props->at(keys[q]).num();
+                } else if (tag == "OrbitingBody") {
+                    r |= POS_ROLE_DISK_ORBITING;
+                }
+            }
             const Value &wind = p.at("wind");
             if (wind.at("effect").is_string() && wind.at("effect").str() == "Interaction") {
                 r |= POS_ROLE_WIND;
@@ -208,8 +221,10 @@ struct Case {
                     tab_radius.push_back(ev->at("radius")[j].num());
                     const Value &rg = ev->at("radius_of_gyration_2");
                     const Value &lv = ev->at("love_number");
+                    const Value *iq = ev->find("inverse_tidal_q_factor");
                     tab_rg2.push_back(rg.items.size() == len ? rg[j].num() : 0.);
                     tab_love.push_back(lv.items.size() == len ? lv[j].num() : 0.);
+                    tab_itq.push_back(iq && iq->items.size() == len ? (*iq)[j].num() : 0.);
                 }
                 evo_left.push_back((int32_t)ev->at("left_index").integer());
                 evo_offset.push_back(evo_offset.back() + (int32_t)len);
@@ -258,6 +273,8 @@ struct Case {
         d.evolver_radius_of_gyration_2 = tab_rg2.data();
         d.evolver_love_number = tab_love.data();
         d.evolver_left_index = evo_left.data();
+        d.evolver_inverse_tidal_q_factor = tab_itq.data();
+        d.disk_properties = disk_properties.data();
         return d;
     }
 };

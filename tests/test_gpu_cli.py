@@ -70,3 +70,40 @@ def test_start_and_resume_write_identical_history(tmp_path):
     cli("resume", str(tmp_path / "b.json"), str(tmp_path / "b.bin"), "-s", "--time-limit",
         repr(full["universe"]["time_limit"]))
     assert open(tmp_path / "b.bin", "rb").read() == open(tmp_path / "a.bin", "rb").read()
+
+
+@pytest.mark.parametrize("name", ["test_disk-enabled_disk", "test_evolution-solar_like_bolmontmathis2016",
+                                  "test_general_relativity-newhall1983"])
+def test_start_history_equals_the_oracle(tmp_path, oracle, name):
+    """A reference case run to its time limit by the command-line host: every
+    historic snapshot (taken at the start of steps 0, 20, ..., 180) holds the
+    oracle's state after that many steps, bit for bit, and the records carry
+    the disk migration timescale and the star's lag angle."""
+    from posidonius_amd.case import read_case
+    with open(os.path.join(GOLDEN_DIR, name + ".json")) as f:
+        case = json.load(f)["case"]
+    n = case["universe"]["n_particles"]
+    case["historic_snapshot_period"] = 20 * case["time_step"]
+    with open(tmp_path / "case.json", "w") as f:
+        json.dump(case, f)
+    cli("start", str(tmp_path / "case.json"), str(tmp_path / "s.json"), str(tmp_path / "h.bin"), "-s")
+    rec = read_history(tmp_path / "h.bin", n)
+    assert len(rec) == 10 * n
+    spec = read_case(case)
+    for r in (1, 5, 9):
+        ref = oracle.run(spec, 20 * r)
+        for i in range(n):
+            got = rec[r * n + i]
+            assert got[2] == i
+            assert list(got[3:6]) == list(ref["position"][i]), (r, i)
+            assert list(got[9:12]) == list(ref["velocity"][i]), (r, i)
+    lag = np.array([x[17] for x in rec])            # lag_angle
+    mig = np.array([x[19] for x in rec])            # migration_timescale
+    body = np.array([x[2] for x in rec])
+    later = np.array([x[0] for x in rec]) > 0.0
+    if "disk" in name:
+        assert np.all(mig[later & (body > 0)] > 0) and np.all(mig[body == 0] == 0)
+    elif "bolmont" in name:
+        assert np.all(lag[body == 0] > 0) and np.all(lag[body > 0] == 0)
+    else:
+        assert np.all(lag == 0) and np.all(mig == 0)
