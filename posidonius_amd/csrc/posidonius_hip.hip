@@ -31,6 +31,7 @@ using namespace posdev;
 // ===========================================================================
 namespace {
 
+template <int V = 0>
 __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, int wave) {
     const int lane = threadIdx.x;
     const int G = E.wave_group[wave];
@@ -73,6 +74,10 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
     if (gvalid && evo_dynamical(E.evo_type[b0 + B.host])) f |= F_DYN_HOST;
     if (B.evo == POS_EVO_LECONTE_CHABRIER2013 && (roles & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_TIDES_ORBITING)))
         f |= F_LC_LOVE;
+    if constexpr ((V & VAR_FAST) != 0) {   // the launch holds only systems of the fixed effect set
+        f = (valid ? F_VALID : 0u) | ((valid && k == B.host) ? F_HOST : 0u) | F_TIDES | F_RF | F_GR | F_SPIN;
+        B.coord = POS_COORD_DEMOCRATIC_HELIOCENTRIC;
+    }
     B.flags = f;
     B.dt = E.dt[s];
     B.t = E.t[s];
@@ -158,7 +163,7 @@ void whfast_steps_kernel(DevEnsemble E, int n_steps, int wave_begin, int wave_en
     __shared__ WaveLds lds;
     for (int wave = wave_begin + blockIdx.x; wave < wave_end; wave += gridDim.x) {
         Lane B;
-        load_lane(E, lds, B, wave);
+        load_lane<V>(E, lds, B, wave);
         const TapeRef none{nullptr, 0};
         for (int step = 0; step < n_steps; ++step) {
             if (B.status & FROZEN) break;   // uniform within the group
@@ -242,7 +247,7 @@ __global__ __launch_bounds__(64) void effects_kernel(DevEnsemble E, double *out_
     const int wave = wave_begin + blockIdx.x;
     if (wave >= wave_end) return;
     Lane B;
-    load_lane(E, lds, B, wave);
+    load_lane<V>(E, lds, B, wave);
     double a[3], dL[3];
     additional_effects<V>(E, lds, B, false, a, dL);
     const int nb = E.n_bodies;
@@ -456,6 +461,10 @@ int system_variant(const pos_ensemble_desc *d, int s, int host) {
     }
     const bool dyn_host = d->evolution_type && evo_dynamical(d->evolution_type[d->body_offset[s] + host]);
     if ((effects & POS_EFFECT_DISK) || dyn_host) v |= VAR_EXT;
+    constexpr uint32_t FAST_EFFECTS = POS_EFFECT_TIDES | POS_EFFECT_ROTATIONAL_FLATTENING | POS_EFFECT_GENERAL_RELATIVITY;
+    if (v == GRV_KIDDER && effects == FAST_EFFECTS && implementation == POS_GR_KIDDER1995 &&
+        d->coordinates[s] == POS_COORD_DEMOCRATIC_HELIOCENTRIC)
+        v |= VAR_FAST;
     return v;
 }
 
@@ -605,6 +614,7 @@ int launch_steps(pos_ens *h, int n_steps) {
             case 4: launch_steps_variant<4>(h, grid, n_steps, w0, w1); break;
             case 5: launch_steps_variant<5>(h, grid, n_steps, w0, w1); break;
             case 6: launch_steps_variant<6>(h, grid, n_steps, w0, w1); break;
+            case 8: launch_steps_variant<8>(h, grid, n_steps, w0, w1); break;
             default: return fail(POS_E_INVALID, "internal error: unknown kernel variant");
         }
         HIP_TRY(hipGetLastError());
@@ -1107,6 +1117,7 @@ int32_t pos_additional_effects(pos_ens *h, double *out_acc, double *out_dLdt) {
             case 4: hipLaunchKernelGGL(effects_kernel<4>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
             case 5: hipLaunchKernelGGL(effects_kernel<5>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
             case 6: hipLaunchKernelGGL(effects_kernel<6>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 8: hipLaunchKernelGGL(effects_kernel<8>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
             default: return fail(POS_E_INVALID, "internal error: unknown kernel variant");
         }
         HIP_TRY(hipGetLastError());

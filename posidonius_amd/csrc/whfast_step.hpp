@@ -316,9 +316,14 @@ enum GrVariant { GRV_KIDDER = 0, GRV_ANDERSON = 1, GRV_NEWHALL = 2 };
 // or an evolving-dissipation host (the extended effects pass).  The device
 // copy of a system's effect mask carries its variant in bits 16-18.
 constexpr int VAR_EXT = 4;
-constexpr int N_VARIANTS = 8;
+// VAR_FAST: every system of the launch has exactly tides + flattening + Kidder
+// GR (spins integrated), no wind, disk or evolution, democratic heliocentric
+// coordinates; the flags and the coordinate type are then compile-time
+// constants and the per-system branches fold away (same arithmetic).
+constexpr int VAR_FAST = 8;
+constexpr int N_VARIANTS = 16;
 constexpr uint32_t EFF_VAR_SHIFT = 16;
-__host__ __device__ __forceinline__ int eff_variant(uint32_t eff) { return (int)((eff >> EFF_VAR_SHIFT) & 7u); }
+__host__ __device__ __forceinline__ int eff_variant(uint32_t eff) { return (int)((eff >> EFF_VAR_SHIFT) & 15u); }
 __host__ __device__ __forceinline__ int eff_gr_variant(uint32_t eff) { return eff_variant(eff) & 3; }
 
 // Chained order of general_relativity.rs (host, then the other bodies by
