@@ -74,8 +74,11 @@ __device__ __forceinline__ double p10(double x) { double x2 = x * x; double x4 =
 // quotient RN(a/b) (Markstein's theorem; Muller et al., Handbook of
 // Floating-Point Arithmetic, 2nd ed., sec. 4.7), barring overflow and
 // underflow, which the step's magnitudes are far from.  The result is thus
-// bit-identical to a/b at about a third of the instruction cost.  An exact
-// zero residual keeps q0, which preserves the sign of a zero quotient.
+// bit-identical to a/b at about a third of the instruction cost.  The sign is
+// taken from q0: RN(a r) has the sign of a/b (also when it underflows to
+// zero), and so does the corrected quotient, except that q0 = -0 with a zero
+// residual would come back as +0 from the FMA; copysign restores it (one
+// bitfield insert instead of a compare and a select).
 struct Rcp {
     double b, r;
 };
@@ -83,7 +86,7 @@ __device__ __forceinline__ Rcp rcp(double b) { return Rcp{b, 1.0 / b}; }
 __device__ __forceinline__ double divr(double a, const Rcp &R) {
     const double q0 = a * R.r;
     const double e = __builtin_fma(-R.b, q0, a);
-    return e == 0. ? q0 : __builtin_fma(e, R.r, q0);
+    return __builtin_copysign(__builtin_fma(e, R.r, q0), q0);
 }
 
 // ---- device view of an ensemble (structure of arrays, device order) -------
@@ -743,9 +746,10 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         // ---- flattening: orthogonal components (oblate_spheroid.rs:21-49) ----
         double fs = 0., fp = 0., ofs = 0., ofp = 0.;
         if (B.rf()) {
-            fs = m * lds.k2f[hl] * hs2 * p5(hR) / 6.;
+            constexpr Rcp R6{6., 1. / 6.};
+            fs = divr(m * lds.k2f[hl] * hs2 * p5(hR), R6);
             ofs = -6. * fs * rs_star / (hs2 * d5);
-            fp = hm * lds.k2f[lane] * s2 * p5(R) / 6.;
+            fp = divr(hm * lds.k2f[lane] * s2 * p5(R), R6);
             ofp = -6. * fp * rs_planet / (s2 * d5);
         }
         if (B.tides()) {
