@@ -166,6 +166,7 @@ def main():
     ap.add_argument("--adjoint-segment", type=int, default=1000)
     ap.add_argument("--group-size", type=int, default=0)
     ap.add_argument("--steps-per-launch", type=int, default=0)
+    ap.add_argument("--no-graph", action="store_true", help="launch the K-step kernels directly, without a hipGraph")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB mebibytes five times "
                          "(reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -204,7 +205,7 @@ def main():
     # ---- batched path -----------------------------------------------------
     log("creating the batched ensemble")
     ens = Ensemble(spec, mapping="packed", group_size=args.group_size,
-                   steps_per_launch=args.steps_per_launch, use_graph=True)
+                   steps_per_launch=args.steps_per_launch, use_graph=not args.no_graph)
     if not args.no_tune:
         log("autotuning the launch mapping")
         o = ens.tune(probe_steps=32)
@@ -284,7 +285,7 @@ def main():
                         if n_sys == defaults[args.config] else f"config {args.config} with {n_sys} systems",
             "systems": n_sys, "mean_bodies": mean_n, "effects": "CTL tides + oblate flattening + Kidder GR + spin",
             "coordinates": "Jacobi" if args.config == 3 else "DemocraticHeliocentric",
-            "group_size": o.group_size, "steps_per_launch": K, "hip_graph": True,
+            "group_size": o.group_size, "steps_per_launch": K, "hip_graph": bool(o.use_graph),
         },
         "roofline": {
             "bound": "mfma",

@@ -259,7 +259,9 @@ int32_t pos_get_options(const pos_ens *ens, pos_options *out);
 
 /* Autotuner: times candidate (group size, steps per launch) mappings on a
  * copy of the ensemble for `probe_steps` steps each, keeps the fastest, and
- * restores the state.  Writes the chosen mapping to *chosen (optional). */
+ * restores the state; when the handle was created with use_graph, it then
+ * times graph replay against direct launches for that mapping and keeps the
+ * faster.  Writes the chosen mapping to *chosen (optional). */
 int32_t pos_tune(pos_ens *ens, int32_t probe_steps, pos_options *chosen);
 
 /* Operator-level entry points: the two sub-boundaries the reference

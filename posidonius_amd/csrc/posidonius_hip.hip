@@ -1201,9 +1201,31 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
             if ((rc = restore())) return rc;
         }
     }
+    // graph replay against direct launches for the chosen mapping: a graph
+    // saves launch overhead on short launches, but its kernel nodes have
+    // measured slower than direct launches of long ones
+    best.use_graph = 0;
+    if (keep_graph) {
+        double t_mode[2] = {0., 0.};
+        for (int mode = 0; mode < 2; ++mode) {
+            pos_options o = best;
+            o.use_graph = mode;
+            if ((rc = build_launch(h, o))) return rc;
+            const int steps = GRAPH_LAUNCHES * o.steps_per_launch;
+            if ((rc = pos_step(h, steps))) return rc;   // warm-up (captures the graph)
+            HIP_TRY(hipEventRecord(e0, h->stream));
+            if ((rc = pos_step(h, steps))) return rc;
+            HIP_TRY(hipEventRecord(e1, h->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            t_mode[mode] = ms;
+            if ((rc = restore())) return rc;
+        }
+        best.use_graph = t_mode[1] < t_mode[0] ? 1 : 0;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    best.use_graph = keep_graph;
     if ((rc = build_launch(h, best))) return rc;
     if (chosen) *chosen = h->opts;
     return POS_OK;

@@ -428,7 +428,7 @@ int main(int argc, char **argv) {
     opts.device = device;
     opts.mapping = POS_MAPPING_PACKED;
     opts.steps_per_launch = 256;
-    opts.use_graph = 1;
+    opts.use_graph = 0;   // long launches: direct launches measure faster than graph replays
     pos_ens *ens = nullptr;
     check(pos_create(&desc, &opts, &ens), "pos_create");
     check(pos_history_enable(ens, capacity), "pos_history_enable");
