@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define POS_ABI_VERSION 2
+#define POS_ABI_VERSION 3
 
 /* Largest body count per system.  The reference caps a universe at
  * MAX_PARTICLES = 10 (src/constants.rs:3); the uneven-size ensemble needs 12.
@@ -173,6 +173,12 @@ typedef struct pos_ensemble_desc {
      * lifetime (days), alpha, surface_density_normalization,
      * mean_molecular_weight */
     const double   *disk_properties;
+    /* the pair-dependent dissipation map entries (optional; NaN or NULL =
+     * no entry): for an orbiting body with evolving dissipation, its own
+     * factor in its pair with the tidal host, the entry
+     * id * MAX_PARTICLES + host_id of universe.pair_dependent_scaled_
+     * dissipation_factor (constant_time_lag.rs:96-165, 188-194) */
+    const double   *pair_dissipation_factor;   /* [nb] */
 } pos_ensemble_desc;
 
 /*
@@ -197,6 +203,8 @@ typedef struct pos_state {
     uint32_t *status;                /* [n_systems] POS_STATUS_* bits         */
     int32_t  *evolver_left_index;    /* [nb] Evolver.left_index (evolution.rs:26) */
     int32_t  *timestep_warning;      /* [n_systems] WHFast.timestep_warning   */
+    double   *pair_dissipation_factor; /* [nb] map entry of an evolving-dissipation
+                                          orbiting body, NaN = none (see desc) */
 } pos_state;
 
 /* Launch mapping chosen by the autotuner or by the caller. */

@@ -5,6 +5,7 @@
 // pos_ensemble_desc / pos_state structures as the product library so that
 // both sides read identical inputs.
 #include <atomic>
+#include <limits>
 #include <chrono>
 #include <string>
 #include <thread>
@@ -37,6 +38,8 @@ template <class R> void store_state(const oracle::System<R> &S, int b0, int s, p
         if (o->radius_of_gyration_2) o->radius_of_gyration_2[b] = val(P.rg2);
         if (o->moment_of_inertia) o->moment_of_inertia[b] = val(P.inertia);
         if (o->evolver_left_index) o->evolver_left_index[b] = (int32_t)S.ev[i].left_index;
+        if (o->pair_dissipation_factor)
+            o->pair_dissipation_factor[b] = P.ph_set ? val(P.ph) : std::numeric_limits<double>::quiet_NaN();
     }
     if (o->timestep_warning) o->timestep_warning[s] = S.timestep_warning;
     if (o->current_time) o->current_time[s] = val(S.t);

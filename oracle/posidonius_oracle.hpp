@@ -34,9 +34,9 @@
 //   * a system must have every effect host equal to its most massive body,
 //     and when tides / flattening are on every other body must be an
 //     orbiting body of that effect.  Other layouts are rejected at load;
-//   * evolving dissipation (BolmontMathis2016, GalletBolmont2017,
-//     LeconteChabrier2013 with dynamical tides) is accepted for the tidal
-//     host only; the pair-dependent factor map starts empty;
+//   * the pair-dependent factor map holds only what the step reads: the
+//     host's entries are recomputed before every use, the orbiting bodies'
+//     entries persist (loaded from and returned to the caller);
 //   * exp and powf are correctly rounded (oracle_types.hpp), where the
 //     reference's libm is within 0.52 ulp.
 #pragma once
@@ -193,6 +193,9 @@ template <class R> struct Body {
     R sigma_scale, sigma_factor;   // its two factors (ConstantTimeLagParameters)
     R lag;        // lag angle of evolving dissipation (evolution.rs:548-567)
     R sigma_host; // the host's factor in this body's pair (constant_time_lag.rs:20-94, 196-204)
+    R sigma_pair; // this body's own factor in that pair (constant_time_lag.rs:96-165, 196-204)
+    bool ph_set;  // the map entry id * MAX_PARTICLES + host_id exists (persistent)
+    R ph;         // its value
     R k2;         // CTL love number
     R k2f;        // oblate-spheroid (fluid) love number
     R wind_k, wind_sat, wind_sat2;
@@ -294,6 +297,10 @@ inline bool load_system(const pos_ensemble_desc *d, int s, System<R> &S, std::st
         P.sigma_scale = R(d->tides_dissipation_factor_scale[b]);
         P.sigma_factor = R(d->tides_dissipation_factor[b]);
         P.sigma = P.sigma_scale * P.sigma_factor;
+        if (d->pair_dissipation_factor && !std::isnan(d->pair_dissipation_factor[b])) {
+            P.ph_set = true;
+            P.ph = R(d->pair_dissipation_factor[b]);
+        }
         P.k2 = R(d->tides_love_number[b]);
         P.k2f = R(d->flattening_love_number[b]);
         P.wind_k = R(d->wind_k_factor ? d->wind_k_factor[b] : 0.);
@@ -307,10 +314,7 @@ inline bool load_system(const pos_ensemble_desc *d, int s, System<R> &S, std::st
         E = Evolver();
         if (d->evolution_type) {
             E.type = d->evolution_type[b];
-            if (E.dynamical() && i != S.host) {
-                if (err) *err = "evolving dissipation of an orbiting body is not supported";
-                return false;
-            }
+
             if (E.type != POS_EVO_NON_EVOLVING) {
                 const int o = d->evolver_offset[b];
                 E.len = d->evolver_offset[b + 1] - o;
@@ -500,34 +504,63 @@ template <class R> inline void perihelion_distance_and_eccentricity(const R &gm,
     q = s / (R(1.) + e);
 }
 
-// constant_time_lag.rs:20-94: the host's dissipation factor as seen by each
-// orbiting body.  With a host of an evolving-dissipation model whose dynamical
-// tide is excited by the body's mean motion, the pair value; otherwise the
-// host's own scaled factor (the map entry is removed, 196-204).  The
-// host-dependent half (96-165) applies to orbiting bodies of those models,
-// which are rejected at load.
+// constant_time_lag.rs:20-194: the dissipation factors of each host-body pair.
+// A host with an evolving-dissipation model gets, per orbiting body whose
+// mean motion excites its dynamical tide, the pair value (20-94); an
+// orbiting body with such a model gets its own pair value when its spin is
+// excited by the orbit (96-165).  The map of these values: the host's
+// entries are set or removed in every evaluation, and removed again when an
+// evolving orbiting body is not excited (the reference removes the host's
+// key there, 157-160); the orbiting body's entries are never removed.  A
+// missing entry means the body's own scaled factor (188-194).
+template <class R>
+inline bool excited_pair_factor(const R &gm, const V3<R> &x, const V3<R> &u, const R &norm_spin, const R &radius,
+                                const R &lag, const R &scale, const R &factor, R &value) {
+    R q, e;
+    perihelion_distance_and_eccentricity(gm, x, u, q, e);
+    const R mean_motion = sqrt_r(gm) * pow_r(q / (R(1.0) - e), -1.5);
+    R half = abs_r(norm_spin - mean_motion);
+    if (!(val(half) < val(norm_spin))) return false;
+    if (val(half) < SMOOTHING_FACTOR_DYN_TIDE_COROTATION) half = R(SMOOTHING_FACTOR_DYN_TIDE_COROTATION);
+    const R inverse_of_half = R(1.) / half;
+    value = scale * (R(2.0 * K2) / (R(3.0) * pw5(radius)) * lag * inverse_of_half + factor);
+    return true;
+}
+
 template <class R> inline void pair_dependent_dissipation_factors(System<R> &S) {
     const Body<R> &H = S.p[S.host];
-    for (int i = 0; i < S.n; ++i) {
-        if (i == S.host) continue;
-        S.p[i].sigma_host = H.sigma;
+    const bool host_dyn = S.ev[S.host].dynamical();
+    bool hp_set[MAXB];
+    R hp[MAXB];
+    for (int i = 0; i < S.n; ++i) hp_set[i] = false;
+    if (host_dyn) {   // planet-dependent factors of the host (20-94)
+        const R host_norm_spin = sqrt_r(H.spin2);
+        for (int i = 0; i < S.n; ++i) {
+            if (i == S.host) continue;
+            const Body<R> &P = S.p[i];
+            if (!(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
+            hp_set[i] = excited_pair_factor(H.mg + P.mg, P.hx, P.hv, host_norm_spin, H.radius, H.lag, H.sigma_scale,
+                                            H.sigma_factor, hp[i]);
+        }
     }
-    if (!S.ev[S.host].dynamical()) return;
-    const R host_norm_spin = sqrt_r(H.spin2);
+    for (int i = 0; i < S.n; ++i) {   // host-dependent factors of the orbiting bodies (96-165)
+        if (i == S.host) continue;
+        Body<R> &P = S.p[i];
+        if (!S.ev[i].dynamical() || !(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
+        R v;
+        if (excited_pair_factor(H.mg + P.mg, P.hx, P.hv, sqrt_r(P.spin2), P.radius, P.lag, P.sigma_scale,
+                                P.sigma_factor, v)) {
+            P.ph = v;
+            P.ph_set = true;
+        } else {
+            hp_set[i] = false;
+        }
+    }
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
-        if (!(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
-        const R gm = H.mg + P.mg;
-        R q, e;
-        perihelion_distance_and_eccentricity(gm, P.hx, P.hv, q, e);
-        const R mean_motion = sqrt_r(gm) * pow_r(q / (R(1.0) - e), -1.5);
-        R half = abs_r(host_norm_spin - mean_motion);
-        if (val(half) < val(host_norm_spin)) {
-            if (val(half) < SMOOTHING_FACTOR_DYN_TIDE_COROTATION) half = R(SMOOTHING_FACTOR_DYN_TIDE_COROTATION);
-            const R inverse_of_half = R(1.) / half;
-            P.sigma_host = H.sigma_scale * (R(2.0 * K2) / (R(3.0) * pw5(H.radius)) * H.lag * inverse_of_half + H.sigma_factor);
-        }
+        P.sigma_host = (host_dyn && hp_set[i]) ? hp[i] : H.sigma;
+        P.sigma_pair = (S.ev[i].dynamical() && P.ph_set) ? P.ph : P.sigma;
     }
 }
 
@@ -544,7 +577,7 @@ template <class R> inline void tides_orthogonal(System<R> &S) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
         R d7 = pw7(P.hdist);
-        P.td_orth_planet = R(4.5) * pw2(H.m) * pw10(P.radius) * P.sigma / d7;
+        P.td_orth_planet = R(4.5) * pw2(H.m) * pw10(P.radius) * P.sigma_pair / d7;
     }
 }
 // oblate_spheroid.rs:8-49
@@ -575,7 +608,7 @@ template <class R> inline void tides_radial(System<R> &S) {
         R conservative = R(-3.0 * K2) / pw7(P.hdist) * (pm2 * pw5(H.radius) * H.k2 + host_mass_2 * pw5(P.radius) * P.k2);
         R factor1 = R(-13.5) * P.hvr / pw8(P.hdist);
         R term1 = pm2 * pw10(H.radius) * P.sigma_host;
-        R term2 = host_mass_2 * pw10(P.radius) * P.sigma;
+        R term2 = host_mass_2 * pw10(P.radius) * P.sigma_pair;
         P.td_diss_pm = factor1 * term2;
         R dissipative = P.td_diss_pm + factor1 * term1;
         P.td_radial = conservative + dissipative;

@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_BODIES = 16
 
 OK = 0
@@ -122,6 +122,7 @@ class EnsembleDesc(C.Structure):
         ("evolver_left_index", _pi32),
         ("evolver_inverse_tidal_q_factor", _pd),
         ("disk_properties", _pd),
+        ("pair_dissipation_factor", _pd),
     ]
 
 
@@ -143,6 +144,7 @@ class State(C.Structure):
         ("status", _pu32),
         ("evolver_left_index", _pi32),
         ("timestep_warning", _pi32),
+        ("pair_dissipation_factor", _pd),
     ]
 
 
@@ -200,6 +202,7 @@ STATE_FIELDS = {
     "status": (0, np.uint32),
     "evolver_left_index": (1, np.int32),
     "timestep_warning": (0, np.int32),
+    "pair_dissipation_factor": (1, np.float64),
 }
 
 _CTYPE = {np.float64: C.c_double, np.int64: C.c_int64, np.uint32: C.c_uint32, np.int32: C.c_int32}

@@ -36,8 +36,11 @@ BODY_FIELDS = {
     "flattening_love_number": (np.float64, 1), "wind_k_factor": (np.float64, 1),
     "wind_rotation_saturation": (np.float64, 1), "velocity_error": (np.float64, 3),
     "angular_momentum_error": (np.float64, 3), "evolution_type": (np.int32, 1),
-    "evolver_left_index": (np.int32, 1),
+    "evolver_left_index": (np.int32, 1), "pair_dissipation_factor": (np.float64, 1),
 }
+# fill value of a per-body field a spec does not carry (NaN: no map entry)
+BODY_FILL = {"pair_dissipation_factor": np.nan}
+MAX_PARTICLES = 10   # src/constants.rs:3, the key base of the pair-dependent factor map
 TABLE_FIELDS = ("evolver_time", "evolver_radius", "evolver_radius_of_gyration_2", "evolver_love_number",
                 "evolver_inverse_tidal_q_factor")
 # Per-system fields wider than one value: DiskProperties of the disk host
@@ -103,7 +106,7 @@ class EnsembleSpec:
             filled = []
             for s, p in zip(specs, parts):
                 if p is None:
-                    p = np.zeros((s.n_bodies, 3) if w == 3 else s.n_bodies, dtype=dt)
+                    p = np.full((s.n_bodies, 3) if w == 3 else s.n_bodies, BODY_FILL.get(k, 0), dtype=dt)
                 filled.append(p)
             out.body[k] = np.concatenate(filled).astype(dt)
         if any(s.evolver_offset is not None for s in specs):
@@ -324,6 +327,16 @@ def read_case(path_or_dict) -> EnsembleSpec:
         else:
             body["evolver_left_index"].append(0)
             eo.append(eo[-1])
+    # pair-dependent dissipation map (constant_time_lag.rs:188-194): the
+    # entries id * MAX_PARTICLES + host_id of orbiting bodies
+    pairs = np.full(n, np.nan)
+    ids = [int(p.get("id", i)) for i, p in enumerate(u["particles"][:n])]
+    host = int(np.argmax(body["mass"])) if n else 0
+    for key, value in (u.get("pair_dependent_scaled_dissipation_factor") or {}).items():
+        owner, dep = divmod(int(key), MAX_PARTICLES)
+        if dep == ids[host] and owner in ids and ids.index(owner) != host:
+            pairs[ids.index(owner)] = float(value)
+    body["pair_dissipation_factor"] = pairs
     for k, (dt, w) in BODY_FIELDS.items():
         a = np.asarray(body[k], dtype=dt)
         spec.body[k] = a.reshape(n, 3) if w == 3 else a

@@ -72,7 +72,8 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
     if (roles & POS_ROLE_DISK_ORBITING) f |= F_DISK_ROLE;
     if (evo_dynamical(B.evo)) f |= F_DYN;
     if (gvalid && evo_dynamical(E.evo_type[b0 + B.host])) f |= F_DYN_HOST;
-    if (B.evo == POS_EVO_LECONTE_CHABRIER2013 && (roles & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_TIDES_ORBITING)))
+    if ((B.evo == POS_EVO_LECONTE_CHABRIER2013 || B.evo == POS_EVO_LECONTE_CHABRIER2013_DYN) &&
+        (roles & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_TIDES_ORBITING)))
         f |= F_LC_LOVE;
     if constexpr ((V & VAR_FAST) != 0) {   // the launch holds only systems of the fixed effect set
         f = (valid ? F_VALID : 0u) | ((valid && k == B.host) ? F_HOST : 0u) | F_TIDES | F_RF | F_GR | F_SPIN;
@@ -459,8 +460,11 @@ int system_variant(const pos_ensemble_desc *d, int s, int host) {
         if (implementation == POS_GR_ANDERSON1975) v = GRV_ANDERSON;
         if (implementation == POS_GR_NEWHALL1983) v = GRV_NEWHALL;
     }
-    const bool dyn_host = d->evolution_type && evo_dynamical(d->evolution_type[d->body_offset[s] + host]);
-    if ((effects & POS_EFFECT_DISK) || dyn_host) v |= VAR_EXT;
+    (void)host;
+    bool dyn = false;   // an evolving-dissipation body (host or orbiting)
+    if (d->evolution_type)
+        for (int b = d->body_offset[s]; b < d->body_offset[s + 1]; ++b) dyn = dyn || evo_dynamical(d->evolution_type[b]);
+    if ((effects & POS_EFFECT_DISK) || dyn) v |= VAR_EXT;
     constexpr uint32_t FAST_EFFECTS = POS_EFFECT_TIDES | POS_EFFECT_ROTATIONAL_FLATTENING | POS_EFFECT_GENERAL_RELATIVITY;
     if (v == GRV_KIDDER && effects == FAST_EFFECTS && implementation == POS_GR_KIDDER1995 &&
         d->coordinates[s] == POS_COORD_DEMOCRATIC_HELIOCENTRIC)
@@ -539,8 +543,6 @@ int validate(const pos_ensemble_desc *d) {
                 const int et = d->evolution_type[b0 + i];
                 if (et < POS_EVO_NON_EVOLVING || et > POS_EVO_GALLET_BOLMONT2017)
                     return fail(POS_E_INVALID, where + "bad evolution type");
-                if (evo_dynamical(et) && i != mm)
-                    return fail(POS_E_UNSUPPORTED, where + "evolving dissipation of an orbiting body is not supported");
                 if (evo_dynamical(et) && !d->evolver_inverse_tidal_q_factor)
                     return fail(POS_E_INVALID, where + "evolving dissipation needs evolver_inverse_tidal_q_factor");
                 if (et != POS_EVO_NON_EVOLVING &&
@@ -706,6 +708,7 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     std::vector<double> x(3 * nb), v(3 * nb), L(3 * nb), sp(3 * nb), ev(3 * nb, 0.), eL(3 * nb, 0.), acc(3 * nb, 0.);
     std::vector<double> m(nb), R(nb), rg2(nb), I(nb), sigma(nb), k2(nb), k2f(nb), wk(nb, 0.), ws(nb, 0.);
     std::vector<double> sig_scale(nb), sig_factor(nb), zeros_nb(nb, 0.);
+    std::vector<double> pair(nb, std::numeric_limits<double>::quiet_NaN());
     std::vector<uint32_t> roles(nb);
     std::vector<int> evo_type(nb, POS_EVO_NON_EVOLVING), evo_off(nb, 0), evo_len(nb, 0), left(nb, 0);
     std::vector<double> tab_t, tab_R, tab_rg2, tab_love, tab_itq;
@@ -718,6 +721,7 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
         sigma[db] = d->tides_dissipation_factor_scale[b] * d->tides_dissipation_factor[b];
         sig_scale[db] = d->tides_dissipation_factor_scale[b];
         sig_factor[db] = d->tides_dissipation_factor[b];
+        if (d->pair_dissipation_factor) pair[db] = d->pair_dissipation_factor[b];
         k2[db] = d->tides_love_number[b];
         k2f[db] = d->flattening_love_number[b];
         if (d->wind_k_factor) wk[db] = d->wind_k_factor[b];
@@ -757,7 +761,7 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     double *d_dt, *d_tl, *d_t, *d_lh, *d_hp;
     long long *d_iter;
     double *d_x, *d_v, *d_L, *d_sp, *d_ev, *d_eL, *d_acc, *d_R, *d_rg2, *d_I, *d_m, *d_sigma, *d_k2, *d_k2f, *d_wk, *d_ws, *d_roche;
-    double *d_tt, *d_tR, *d_trg2, *d_tlove, *d_titq, *d_sscale, *d_sfactor, *d_lag, *d_mig, *d_disk;
+    double *d_tt, *d_tR, *d_trg2, *d_tlove, *d_titq, *d_sscale, *d_sfactor, *d_lag, *d_mig, *d_disk, *d_pair;
     if ((rc = dalloc(h, &d_bo, ns + 1)) || (rc = dalloc(h, &d_host, ns)) || (rc = dalloc(h, &d_coord, ns)) ||
         (rc = dalloc(h, &d_tw, ns)) || (rc = dalloc(h, &d_eff, ns)) || (rc = dalloc(h, &d_status, ns)) ||
         (rc = dalloc(h, &d_dt, ns)) || (rc = dalloc(h, &d_tl, ns)) || (rc = dalloc(h, &d_t, ns)) ||
@@ -772,7 +776,8 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
         (rc = dalloc(h, &d_left, nb)) || (rc = dalloc(h, &d_tt, tab_t.size())) || (rc = dalloc(h, &d_tR, tab_t.size())) ||
         (rc = dalloc(h, &d_trg2, tab_t.size())) || (rc = dalloc(h, &d_tlove, tab_t.size())) ||
         (rc = dalloc(h, &d_titq, tab_t.size())) || (rc = dalloc(h, &d_sscale, nb)) || (rc = dalloc(h, &d_sfactor, nb)) ||
-        (rc = dalloc(h, &d_lag, nb)) || (rc = dalloc(h, &d_mig, nb)) || (rc = dalloc(h, &d_disk, 6 * (size_t)ns))) {
+        (rc = dalloc(h, &d_lag, nb)) || (rc = dalloc(h, &d_mig, nb)) || (rc = dalloc(h, &d_disk, 6 * (size_t)ns)) ||
+        (rc = dalloc(h, &d_pair, nb))) {
         pos_destroy(h);
         return rc;
     }
@@ -786,6 +791,7 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     E.evo_type = d_evo_type; E.evo_off = d_evo_off; E.evo_len = d_evo_len; E.left_index = d_left;
     E.tab_time = d_tt; E.tab_radius = d_tR; E.tab_rg2 = d_trg2; E.tab_love = d_tlove; E.tab_itq = d_titq;
     E.sigma_scale = d_sscale; E.sigma_factor = d_sfactor; E.lag = d_lag; E.disk_migration = d_mig; E.disk_props = d_disk;
+    E.pair_ph = d_pair;
 
     if ((rc = upload(h, d_bo, h->body_offset_dev)) || (rc = upload(h, d_host, host_idx)) || (rc = upload(h, d_coord, coord)) ||
         (rc = upload(h, d_tw, tw)) || (rc = upload(h, d_eff, eff)) || (rc = upload(h, d_status, status)) ||
@@ -799,7 +805,8 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
         (rc = upload(h, d_left, left)) || (rc = upload(h, d_tt, tab_t)) || (rc = upload(h, d_tR, tab_R)) ||
         (rc = upload(h, d_trg2, tab_rg2)) || (rc = upload(h, d_tlove, tab_love)) || (rc = upload(h, d_titq, tab_itq)) ||
         (rc = upload(h, d_sscale, sig_scale)) || (rc = upload(h, d_sfactor, sig_factor)) ||
-        (rc = upload(h, d_lag, zeros_nb)) || (rc = upload(h, d_mig, zeros_nb)) || (rc = upload(h, d_disk, disk))) {
+        (rc = upload(h, d_lag, zeros_nb)) || (rc = upload(h, d_mig, zeros_nb)) || (rc = upload(h, d_disk, disk)) ||
+        (rc = upload(h, d_pair, pair))) {
         pos_destroy(h);
         return rc;
     }
@@ -942,6 +949,12 @@ int32_t pos_get_state(pos_ens *h, pos_state *o) {
         HIP_TRY(hipStreamSynchronize(h->stream));
         for (int ds = 0; ds < ns; ++ds) o->timestep_warning[h->sys_perm[ds]] = tw[ds];
     }
+    if (o->pair_dissipation_factor) {
+        std::vector<double> pf;
+        if ((rc = download(h, pf, E.pair_ph, nb))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (int db = 0; db < nb; ++db) o->pair_dissipation_factor[h->body_perm[db]] = pf[db];
+    }
     return POS_OK;
 }
 
@@ -1010,6 +1023,12 @@ int32_t pos_set_state(pos_ens *h, const pos_state *in) {
         std::vector<int> tw(ns);
         for (int ds = 0; ds < ns; ++ds) tw[ds] = in->timestep_warning[h->sys_perm[ds]];
         if ((rc = upload(h, E.timestep_warning, tw))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    if (in->pair_dissipation_factor) {
+        std::vector<double> pf(nb);
+        for (int db = 0; db < nb; ++db) pf[db] = in->pair_dissipation_factor[h->body_perm[db]];
+        if ((rc = upload(h, E.pair_ph, pf))) return rc;
         HIP_TRY(hipStreamSynchronize(h->stream));
     }
     return POS_OK;
@@ -1140,7 +1159,8 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
     // snapshot the mutable state so that probing leaves it untouched
     const int ns = h->n_systems, nb = h->n_bodies;
     std::vector<double> sx(3 * (size_t)nb), sv(3 * (size_t)nb), sL(3 * (size_t)nb), ss(3 * (size_t)nb), sev(3 * (size_t)nb),
-        seL(3 * (size_t)nb), sacc(3 * (size_t)nb), sR(nb), srg2(nb), sI(nb), st(ns), slh(ns);
+        seL(3 * (size_t)nb), sacc(3 * (size_t)nb), sR(nb), srg2(nb), sI(nb), st(ns), slh(ns), slag(nb), smig(nb),
+        spair(nb);
     std::vector<long long> sit(ns);
     std::vector<uint32_t> sst(ns);
     std::vector<int> stw(ns), sleft(nb);
@@ -1153,7 +1173,8 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
         (rc = download(h, srg2, E.rg2, nb)) || (rc = download(h, sI, E.inertia, nb)) || (rc = download(h, st, E.t, ns)) ||
         (rc = download(h, slh, E.last_hist, ns)) || (rc = download(h, sit, E.iteration, ns)) ||
         (rc = download(h, sst, E.status, ns)) || (rc = download(h, stw, E.timestep_warning, ns)) ||
-        (rc = download(h, sleft, E.left_index, nb)))
+        (rc = download(h, sleft, E.left_index, nb)) || (rc = download(h, slag, E.lag, nb)) ||
+        (rc = download(h, smig, E.disk_migration, nb)) || (rc = download(h, spair, E.pair_ph, nb)))
         return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     auto restore = [&]() -> int {
@@ -1162,7 +1183,8 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
             (r = upload(h, E.err_v, sev)) || (r = upload(h, E.err_L, seL)) || (r = upload(h, E.acc, sacc)) ||
             (r = upload(h, E.radius, sR)) || (r = upload(h, E.rg2, srg2)) || (r = upload(h, E.inertia, sI)) ||
             (r = upload(h, E.t, st)) || (r = upload(h, E.last_hist, slh)) || (r = upload(h, E.iteration, sit)) ||
-            (r = upload(h, E.status, sst)) || (r = upload(h, E.timestep_warning, stw)) || (r = upload(h, E.left_index, sleft)))
+            (r = upload(h, E.status, sst)) || (r = upload(h, E.timestep_warning, stw)) || (r = upload(h, E.left_index, sleft)) ||
+            (r = upload(h, E.lag, slag)) || (r = upload(h, E.disk_migration, smig)) || (r = upload(h, E.pair_ph, spair)))
             return r;
         HIP_TRY(hipStreamSynchronize(h->stream));
         return POS_OK;

@@ -117,6 +117,7 @@ struct DevEnsemble {
     // migration timescale of its last disk evaluation (disk.rs:236), and the
     // disk host's DiskProperties per system (6 doubles)
     double *lag, *disk_migration;
+    double *pair_ph;   // an orbiting body's own pair-dependent factor map entry, NaN = none
     const double *disk_props;
     // launch table: wave w covers device systems wave_first[w] .. +wave_count[w]-1
     const int *wave_first, *wave_count, *wave_group;
@@ -230,7 +231,8 @@ __device__ inline void evolve_body(const DevEnsemble &E, WaveLds &lds, Lane &B, 
     const double newR = interp_table(tt + i0, E.tab_radius + off + i0, len - i0, t, &left);
     B.left_index += left;
     double newrg2 = rg2;
-    if (B.evo == POS_EVO_BARAFFE2015 || B.evo == POS_EVO_LECONTE2011 || B.evo == POS_EVO_LECONTE_CHABRIER2013) {
+    if (B.evo == POS_EVO_BARAFFE2015 || B.evo == POS_EVO_LECONTE2011 || B.evo == POS_EVO_LECONTE_CHABRIER2013 ||
+        B.evo == POS_EVO_LECONTE_CHABRIER2013_DYN) {
         i0 = evolver_idx(B.left_index, len);
         newrg2 = interp_table(tt + i0, E.tab_rg2 + off + i0, len - i0, t, &left);
         B.left_index += left;
@@ -597,6 +599,19 @@ __device__ inline void perihelion_distance_and_eccentricity(double gm, const dou
     q = sv / (1. + e);
 }
 
+// constant_time_lag.rs:57-92 / 121-156: the pair value of an evolving-
+// dissipation body (norm_spin, radius, lag angle, sigma's two factors) when
+// the orbit's mean motion excites its dynamical tide.
+__device__ inline bool excited_pair_factor(double mean_motion, double norm_spin, double radius, double lag,
+                                           double scale, double factor, double &value) {
+    double half = fabs(norm_spin - mean_motion);
+    if (!(half < norm_spin)) return false;
+    if (half < SMOOTHING_FACTOR_DYN_TIDE_COROTATION) half = SMOOTHING_FACTOR_DYN_TIDE_COROTATION;
+    const double inverse_of_half = 1. / half;
+    value = scale * (2.0 * K2 / (3.0 * p5(radius)) * lag * inverse_of_half + factor);
+    return true;
+}
+
 // Disk force on an orbiting body (disk.rs:155-268, Alibert et al. 2013) from
 // its heliocentric position and velocity; stores its migration timescale.
 __device__ inline void disk_force(const DevEnsemble &E, const WaveLds &lds, const Lane &B, const double px[3],
@@ -719,29 +734,43 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         const double d7 = p7(d);
         const Rcp Rd7 = rcp(d7);
         const double d5 = p5(d);
-        // ---- the host's dissipation factor in this pair: its own, or with an
-        //      evolving-dissipation host whose dynamical tide this body
-        //      excites, the pair value (constant_time_lag.rs:20-94) ----
-        double sigma_host = lds.sigma[hl];
-        if (EXT && (B.flags & F_DYN_HOST) && B.tides()) {
-            const double host_norm_spin = sqrt(hs2);
+        // ---- the dissipation factors of this host-body pair: the bodies' own
+        //      scaled factors, or for evolving-dissipation models the pair
+        //      values of a dynamical tide this orbit excites
+        //      (constant_time_lag.rs:20-194).  The host's value is recomputed
+        //      here every time; the body's own map entry persists in
+        //      pair_ph, and a body that is not excited removes the host's
+        //      entry, as the reference does (157-160). ----
+        double sigma_host = lds.sigma[hl], sigma_pair = lds.sigma[lane];
+        if (EXT && B.tides() && (B.flags & (F_DYN_HOST | F_DYN))) {
             const double gm = lds.mg[hl] + lds.mg[lane];
             double q, e;
             perihelion_distance_and_eccentricity(gm, px, pv, q, e);
             const double mean_motion = sqrt(gm) * poscr::cr_rsqrt3(q / (1.0 - e));
-            double half = fabs(host_norm_spin - mean_motion);
-            if (half < host_norm_spin) {
-                if (half < SMOOTHING_FACTOR_DYN_TIDE_COROTATION) half = SMOOTHING_FACTOR_DYN_TIDE_COROTATION;
-                const double inverse_of_half = 1. / half;
+            bool host_entry = false;
+            double host_value = 0.;
+            if (B.flags & F_DYN_HOST) {
                 const int hb = B.host_body();
-                sigma_host = E.sigma_scale[hb] * (2.0 * K2 / (3.0 * p5(hR)) * host_lag * inverse_of_half + E.sigma_factor[hb]);
+                host_entry = excited_pair_factor(mean_motion, sqrt(hs2), hR, host_lag, E.sigma_scale[hb],
+                                                 E.sigma_factor[hb], host_value);
             }
+            if (B.flags & F_DYN) {
+                double v;
+                if (excited_pair_factor(mean_motion, sqrt(s2), R, E.lag[B.body], E.sigma_scale[B.body],
+                                        E.sigma_factor[B.body], v))
+                    E.pair_ph[B.body] = v;
+                else
+                    host_entry = false;
+                const double own = E.pair_ph[B.body];
+                if (own == own) sigma_pair = own;   // an entry exists (NaN marks none)
+            }
+            if (host_entry) sigma_host = host_value;
         }
         // ---- tides: orthogonal components (constant_time_lag.rs:206-264) ----
         double os = 0., op = 0.;
         if (B.tides()) {
             os = divr(4.5 * p2(m) * p10(hR) * sigma_host, Rd7);
-            op = divr(4.5 * p2(hm) * p10(R) * lds.sigma[lane], Rd7);
+            op = divr(4.5 * p2(hm) * p10(R) * sigma_pair, Rd7);
         }
         // ---- flattening: orthogonal components (oblate_spheroid.rs:21-49) ----
         double fs = 0., fp = 0., ofs = 0., ofp = 0.;
@@ -759,7 +788,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
             const double conservative = divr(-3.0 * K2, Rd7) * (pm2 * p5(hR) * lds.k2[hl] + hm2 * p5(R) * lds.k2[lane]);
             const double factor1 = -13.5 * vr / p8(d);
             const double term1 = pm2 * p10(hR) * sigma_host;
-            const double term2 = hm2 * p10(R) * lds.sigma[lane];
+            const double term2 = hm2 * p10(R) * sigma_pair;
             const double diss_pm = factor1 * term2;
             const double dissipative = diss_pm + factor1 * term1;
             const double radial = conservative + dissipative;
@@ -1585,8 +1614,10 @@ __device__ inline void write_history_record(const DevEnsemble &E, WaveLds &lds, 
         const double vr = (r[0] * u[0] + r[1] * u[1] + r[2] * u[2]) / d;
         const double hm = lds.m[hl];
         const double R = lds.R[lane];
-        const double op = 4.5 * p2(hm) * p10(R) * lds.sigma[lane] / p7(d);
-        const double diss_pm = -13.5 * vr / p8(d) * (p2(hm) * p10(R) * lds.sigma[lane]);
+        double sigma = lds.sigma[lane];   // or its pair-dependent map entry
+        if (EXT && (B.flags & F_DYN) && E.pair_ph[B.body] == E.pair_ph[B.body]) sigma = E.pair_ph[B.body];
+        const double op = 4.5 * p2(hm) * p10(R) * sigma / p7(d);
+        const double diss_pm = -13.5 * vr / p8(d) * (p2(hm) * p10(R) * sigma);
         const double f2 = op / d;
         const double sp[3] = {lds.s[0][lane], lds.s[1][lane], lds.s[2][lane]};
         const double rs = r[0] * sp[0] + r[1] * sp[1] + r[2] * sp[2];

@@ -31,6 +31,7 @@
 #include <cstring>
 #include <ctime>
 #include <fstream>
+#include <limits>
 #include <iostream>
 #include <memory>
 #include <sstream>
@@ -43,6 +44,8 @@
 using posjson::Value;
 
 namespace {
+
+constexpr long long MAX_PARTICLES = 10;   // src/constants.rs:3, the key base of the pair-dependent map
 
 constexpr int HISTORY_RECORD_BYTES = 156;   // output.rs:58-62
 
@@ -123,6 +126,8 @@ struct Case {
     std::vector<int32_t> evo_type, evo_offset, evo_left, ids;
     std::vector<double> tab_time, tab_radius, tab_rg2, tab_love, tab_itq;
     std::vector<double> disk_properties = std::vector<double>(6, 0.);   // the disk host's DiskProperties
+    std::vector<double> pair_factor;   // per body: its pair-dependent map entry with the host, NaN = none
+    int host = 0;                      // most massive body
 
     explicit Case(Value d) : doc(std::move(d)) {
         if (!doc.find("alternative_coordinates_type")) throw std::runtime_error("not a WHFast case");
@@ -233,6 +238,24 @@ struct Case {
                 evo_offset.push_back(evo_offset.back());
             }
         }
+        read_pair_map(u);
+    }
+
+    // the map entries id * MAX_PARTICLES + host_id (constant_time_lag.rs:188-194)
+    void read_pair_map(const Value &u) {
+        pair_factor.assign(n, std::numeric_limits<double>::quiet_NaN());
+        double mmax = 0.;
+        for (int i = 0; i < n; ++i)
+            if (mass[i] > mmax) { mmax = mass[i]; host = i; }
+        const Value *map = u.find("pair_dependent_scaled_dissipation_factor");
+        if (!map || !map->is_object()) return;
+        for (const auto &m : map->members) {
+            const long long key = std::stoll(m.first);
+            const long long owner = key / MAX_PARTICLES, dep = key % MAX_PARTICLES;
+            if (dep != ids[host]) continue;
+            for (int i = 0; i < n; ++i)
+                if (i != host && ids[i] == owner) pair_factor[i] = m.second.num();
+        }
     }
 
     pos_ensemble_desc desc() const {
@@ -275,6 +298,7 @@ struct Case {
         d.evolver_left_index = evo_left.data();
         d.evolver_inverse_tidal_q_factor = tab_itq.data();
         d.disk_properties = disk_properties.data();
+        d.pair_dissipation_factor = pair_factor.data();
         return d;
     }
 };
@@ -290,12 +314,13 @@ void write_recovery_snapshot(const std::string &path, Case &c, pos_ens *ens, dou
     std::vector<double> x(3 * n), v(3 * n), L(3 * n), s(3 * n), ev(3 * n), eL(3 * n), R(n), rg2(n), I(n), t(1), lh(1);
     std::vector<int64_t> it(1);
     std::vector<int32_t> left(n), tw(1);
+    std::vector<double> pairs(n);
     pos_state st{};
     st.position = x.data(); st.velocity = v.data(); st.angular_momentum = L.data(); st.spin = s.data();
     st.velocity_error = ev.data(); st.angular_momentum_error = eL.data(); st.radius = R.data();
     st.radius_of_gyration_2 = rg2.data(); st.moment_of_inertia = I.data(); st.current_time = t.data();
     st.last_historic_snapshot_time = lh.data(); st.current_iteration = it.data();
-    st.evolver_left_index = left.data(); st.timestep_warning = tw.data();
+    st.evolver_left_index = left.data(); st.timestep_warning = tw.data(); st.pair_dissipation_factor = pairs.data();
     check(pos_get_state(ens, &st), "pos_get_state");
     Value &d = c.doc;
     put_member(d, "current_time", Value::number(t[0]));
@@ -327,6 +352,15 @@ void write_recovery_snapshot(const std::string &path, Case &c, pos_ens *ens, dou
         if (evs && evs->is_array() && (size_t)i < evs->items.size() && (*evs)[i].find("left_index"))
             (*evs)[i].at("left_index") = Value::integer_value(left[i]);
     }
+    // the orbiting bodies' pair-dependent map entries (the host's entries
+    // are recomputed before every use and are not written)
+    Value map;
+    map.kind = Value::Object;
+    for (int i = 0; i < n; ++i)
+        if (!std::isnan(pairs[i]))
+            map.members.emplace_back(std::to_string((long long)c.ids[i] * MAX_PARTICLES + c.ids[c.host]),
+                                     Value::number(pairs[i]));
+    put_member(u, "pair_dependent_scaled_dissipation_factor", std::move(map));
     if (exists(path)) std::rename(path.c_str(), (path + ".previous").c_str());
     std::string out;
     posjson::dump(d, out, 2);

@@ -79,11 +79,12 @@ def test_invalid_desc_is_rejected_before_any_device_work(lib):
     assert b"disk host" in lib.pos_last_error()
 
 
-def test_dynamical_tides_of_an_orbiting_body_are_rejected(lib):
+def test_evolving_dissipation_needs_its_q_factor_table(lib):
     from tests.oracle_runner import load_golden
     spec, _ = load_golden("test_evolution-solar_like_bolmontmathis2016")
-    spec.body["evolution_type"][2] = abi.EVO_LECONTE_CHABRIER2013_DYN
+    spec.body["evolution_type"][2] = abi.EVO_LECONTE_CHABRIER2013_DYN   # an orbiting body: accepted
     h = spec.desc()
+    h.desc.evolver_inverse_tidal_q_factor = abi.ptr(None, C.c_double)
     out = C.c_void_p()
-    assert lib.pos_create(h.ref, None, C.byref(out)) == abi.E_UNSUPPORTED
-    assert b"orbiting body" in lib.pos_last_error()
+    assert lib.pos_create(h.ref, None, C.byref(out)) == abi.E_INVALID
+    assert b"evolver_inverse_tidal_q_factor" in lib.pos_last_error()

@@ -107,3 +107,35 @@ def test_start_history_equals_the_oracle(tmp_path, oracle, name):
         assert np.all(lag[body == 0] > 0) and np.all(lag[body > 0] == 0)
     else:
         assert np.all(lag == 0) and np.all(mig == 0)
+
+
+def test_resume_with_orbiting_dynamical_tides(tmp_path):
+    """An orbiting body with LeconteChabrier2013 dynamical tides: a run stopped
+    early and resumed from its snapshot (which carries the body's
+    pair-dependent map entry, constant_time_lag.rs:188-194) writes the same
+    history as an uninterrupted run."""
+    with open(os.path.join(GOLDEN_DIR, "test_evolution-solar_like_bolmontmathis2016.json")) as f:
+        case = json.load(f)["case"]
+    u = case["universe"]
+    u["particles"][2]["evolution"] = {"LeconteChabrier2013": True}
+    ev = u["particles_evolvers"][2]
+    ev["evolution"] = {"LeconteChabrier2013": True}
+    ev["inverse_tidal_q_factor"] = [0.1] * len(ev["time"])
+    n = u["n_particles"]
+    dt = case["time_step"]
+    case["historic_snapshot_period"] = 10 * dt
+    case["recovery_snapshot_period"] = 50 * dt
+    with open(tmp_path / "full.json", "w") as f:
+        json.dump(case, f)
+    cli("start", str(tmp_path / "full.json"), str(tmp_path / "a.json"), str(tmp_path / "a.bin"), "-s")
+    limit = u["time_limit"]
+    u["time_limit"] = limit * 0.6
+    with open(tmp_path / "short.json", "w") as f:
+        json.dump(case, f)
+    cli("start", str(tmp_path / "short.json"), str(tmp_path / "b.json"), str(tmp_path / "b.bin"), "-s")
+    snap = json.load(open(tmp_path / "b.json"))
+    entries = snap["universe"]["pair_dependent_scaled_dissipation_factor"]
+    assert list(entries) == ["20"]     # body id 2, host id 0
+    cli("resume", str(tmp_path / "b.json"), str(tmp_path / "b.bin"), "-s", "--time-limit", repr(limit))
+    assert open(tmp_path / "b.bin", "rb").read() == open(tmp_path / "a.bin", "rb").read()
+    assert len(read_history(tmp_path / "a.bin", n)) > 0

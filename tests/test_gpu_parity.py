@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 SUPPORTED = golden_names()
 STATE_KEYS = ("position", "velocity", "acceleration", "angular_momentum", "spin", "velocity_error",
               "angular_momentum_error", "radius", "moment_of_inertia", "current_time", "current_iteration",
-              "status", "evolver_left_index", "timestep_warning")
+              "status", "evolver_left_index", "timestep_warning", "pair_dissipation_factor")
 
 
 def log(msg):
@@ -36,7 +36,7 @@ def Ensemble():
 def assert_same_state(gpu, ref, keys=STATE_KEYS, where=""):
     for k in keys:
         a, b = np.asarray(gpu[k]), np.asarray(ref[k])
-        if not np.array_equal(a, b):
+        if not np.array_equal(a, b, equal_nan=a.dtype.kind == "f"):
             bad = np.argwhere(a != b)
             i = tuple(bad[0])
             diff = np.max(np.abs(a.astype(np.float64) - b.astype(np.float64)))
@@ -240,6 +240,19 @@ def excited_dynamical_tide():
     return s
 
 
+def excited_orbiting_dynamical_tide():
+    """The BolmontMathis2016 fixture whose third body (LeconteChabrier2013)
+    has dynamical tides with an inverse tidal Q factor of 0.1: its spin is
+    excited by its orbit and its own pair factor (constant_time_lag.rs:96-165)
+    becomes ~290 times its scaled factor."""
+    spec, _ = load_golden("test_evolution-solar_like_bolmontmathis2016")
+    s = spec.copy()
+    s.body["evolution_type"][2] = abi.EVO_LECONTE_CHABRIER2013_DYN
+    b0, b1 = s.evolver_offset[2], s.evolver_offset[3]
+    s.tables["evolver_inverse_tidal_q_factor"][b0:b1] = 0.1
+    return s
+
+
 def extended_ensemble():
     """Systems of the extended effects pass (the disk and evolving dissipation)
     among plain ones: the three reference fixtures, the disk fixture with its
@@ -250,7 +263,8 @@ def extended_ensemble():
                                          "test_evolution-solar_like_galletbolmont2017")]
     thin = parts[0].copy()
     thin.system["disk_properties"][0, 4] *= 1e-7
-    parts += [thin, excited_dynamical_tide(), gen.config2(16, seed=31), gen.config3(8, seed=32)]
+    parts += [thin, excited_dynamical_tide(), excited_orbiting_dynamical_tide(), gen.config2(16, seed=31),
+              gen.config3(8, seed=32)]
     spec = EnsembleSpec.concat(parts)
     return spec.subset(np.random.default_rng(9).permutation(spec.n_systems))
 
@@ -332,3 +346,29 @@ def test_order_invariance_star_swapped(Ensemble, name):
     for key in ("position", "velocity", "acceleration"):
         np.testing.assert_array_equal(out[key][:n], expected[key])
         np.testing.assert_allclose(out[key][n:][perm], expected[key], rtol=0, atol=1e-16)
+
+
+def test_resume_carries_the_pair_dependent_map(Ensemble):
+    """An evolving-dissipation run stopped, rebuilt from its state (including
+    the orbiting body's pair-dependent map entry and the evolvers'
+    left_index) and continued equals the uninterrupted run bit for bit."""
+    spec = excited_orbiting_dynamical_tide()
+    with Ensemble(spec) as ens:
+        ens.step(100)
+        mid = ens.get_state()
+        ens.step(99)
+        full = ens.get_state()
+    assert not np.isnan(mid["pair_dissipation_factor"][2])
+    resumed = spec.copy()
+    for k in ("position", "velocity", "angular_momentum", "spin", "velocity_error", "angular_momentum_error",
+              "radius", "radius_of_gyration_2", "moment_of_inertia", "evolver_left_index", "pair_dissipation_factor"):
+        resumed.body[k] = mid[k].copy()
+    for k in ("current_time", "last_historic_snapshot_time", "current_iteration"):
+        resumed.system[k] = mid[k].copy()
+    with Ensemble(resumed) as ens2:
+        ens2.set_state({"acceleration": mid["acceleration"]})
+        ens2.step(99)
+        out = ens2.get_state()
+    assert_same_state(out, full, keys=("position", "velocity", "angular_momentum", "spin", "velocity_error",
+                                       "angular_momentum_error", "current_iteration", "evolver_left_index",
+                                       "pair_dissipation_factor"))

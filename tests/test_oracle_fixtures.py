@@ -61,12 +61,26 @@ def test_evolving_dissipation_fixtures_stay_in_the_equilibrium_regime(oracle):
         np.testing.assert_array_equal(oracle.run(plain, 10**9)["position"], expected["position"])
 
 
-def test_oracle_rejects_dynamical_tides_of_an_orbiting_body(oracle):
-    spec, _ = load_golden("test_evolution-solar_like_bolmontmathis2016")
-    bad = spec.copy()
-    bad.body["evolution_type"][2] = abi.EVO_LECONTE_CHABRIER2013_DYN
-    with pytest.raises(RuntimeError, match="evolving dissipation of an orbiting body"):
-        oracle.run(bad, n_steps=1)
+def test_orbiting_body_dynamical_tide_reduces_to_its_own_factor(oracle):
+    """An orbiting body with LeconteChabrier2013 dynamical tides
+    (constant_time_lag.rs:96-165): with a zero inverse tidal Q factor its lag
+    angle is 0 and the pair value scale * (2 K2 / (3 R^5) * 0 / w + factor)
+    is exactly its own scaled factor, so the fixture's bits are unchanged
+    while the map entry is created; with Q^-1 = 0.1 the entry is ~290 times
+    larger and the state moves."""
+    spec, expected = load_golden("test_evolution-solar_like_bolmontmathis2016")
+    dyn = spec.copy()
+    dyn.body["evolution_type"][2] = abi.EVO_LECONTE_CHABRIER2013_DYN
+    out = oracle.run(dyn, 10**9)
+    np.testing.assert_array_equal(out["position"], expected["position"])
+    sigma = spec.body["tides_dissipation_factor"][2] * spec.body["tides_dissipation_factor_scale"][2]
+    assert out["pair_dissipation_factor"][2] == sigma
+    assert np.all(np.isnan(np.delete(out["pair_dissipation_factor"], 2)))
+    b0, b1 = dyn.evolver_offset[2], dyn.evolver_offset[3]
+    dyn.tables["evolver_inverse_tidal_q_factor"][b0:b1] = 0.1
+    out = oracle.run(dyn, 10**9)
+    assert out["pair_dissipation_factor"][2] > 100 * sigma
+    assert np.abs(out["position"] - expected["position"]).max() > 0
 
 
 def _swap_first_two(spec):
