@@ -102,7 +102,13 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
     }
     // persistent LDS area
     lds.m[lane] = valid ? B.m : 0.;
-    lds.mg[lane] = valid ? B.m * K2 : 0.;
+    {
+        const double hm = E.mass[b0 + B.host];
+        const double hmg = hm * K2, pmg = B.m * K2;
+        const bool orbiting = valid && k != B.host;
+        lds.kf[lane] = orbiting ? hmg * pmg / p2(hmg + pmg) : 0.;
+        lds.inv_M[lane] = orbiting ? 1. / (hm + B.m) : 0.;
+    }
     lds.inv_m[lane] = B.inv_m;
     lds.sigma[lane] = E.sigma[b];
     lds.k2[lane] = E.k2[b];

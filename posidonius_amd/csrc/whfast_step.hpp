@@ -147,7 +147,11 @@ __host__ __device__ __forceinline__ bool evo_dynamical(int type) {
 constexpr int NX = 24;   // exchange slots per lane
 struct WaveLds {
     // persistent per body (lane)
-    double m[WAVE], mg[WAVE], inv_m[WAVE], sigma[WAVE], k2[WAVE], k2f[WAVE];
+    double m[WAVE], inv_m[WAVE], sigma[WAVE], k2[WAVE], k2f[WAVE];
+    // of an orbiting body's pair with its host (masses only): the Kidder mass
+    // factor f = G m_h G m / (G m_h + G m)^2 (general_relativity.rs:91) and
+    // 1 / (m_h + m) (general_relativity.rs:326)
+    double kf[WAVE], inv_M[WAVE];
     double R[WAVE], I[WAVE];
     double s[3][WAVE], s2[WAVE];   // spin of the last evaluation and its squared norm
     double ag[3][WAVE];            // gravitational acceleration (universe.rs:300-302)
@@ -156,6 +160,9 @@ struct WaveLds {
 };
 
 __device__ __forceinline__ void wsync() { __syncthreads(); }
+
+// G m of the body in LDS slot l (the reference's mass_g = mass * K2)
+__device__ __forceinline__ double mg_of(const WaveLds &lds, int l) { return lds.m[l] * K2; }
 
 // ---- per-lane registers -----------------------------------------------------
 enum : uint32_t {
@@ -418,7 +425,7 @@ __device__ inline void gr_anderson1975(WaveLds &lds, Lane &B, double g[3]) {
     const double mtot = eta;
     wsync();
     if (B.orbiting()) {
-        const double mu = lds.mg[hl];
+        const double mu = mg_of(lds, hl);
         double vi[3] = {jv[0], jv[1], jv[2]};
         double vi2 = p2(jv[0]) + p2(jv[1]) + p2(jv[2]);
         const double ri = sqrt(p2(jx[0]) + p2(jx[1]) + p2(jx[2]));
@@ -618,7 +625,7 @@ __device__ inline void disk_force(const DevEnsemble &E, const WaveLds &lds, cons
                                   const double pv[3], double d, double nv2, double F[3]) {
     const double *dp = E.disk_props + 6 * B.sys;   // inner, outer, lifetime, alpha, sigma0, mu
     const int hl = B.hl();
-    const double hm = lds.m[hl], hmg = lds.mg[hl], m = B.m, mg = lds.mg[B.lane];
+    const double hm = lds.m[hl], hmg = mg_of(lds, hl), m = B.m, mg = mg_of(lds, B.lane);
     const double sma = (mg + hmg) * d / (2.0 * (mg + hmg) - d * nv2);
     // gas radial velocity from the alpha viscosity (disk.rs:206-226)
     const double keplerian_frequency = sqrt(hmg / (d * (d * d)));
@@ -743,7 +750,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         //      entry, as the reference does (157-160). ----
         double sigma_host = lds.sigma[hl], sigma_pair = lds.sigma[lane];
         if (EXT && B.tides() && (B.flags & (F_DYN_HOST | F_DYN))) {
-            const double gm = lds.mg[hl] + lds.mg[lane];
+            const double gm = mg_of(lds, hl) + mg_of(lds, lane);
             double q, e;
             perihelion_distance_and_eccentricity(gm, px, pv, q, e);
             const double mean_motion = sqrt(gm) * poscr::cr_rsqrt3(q / (1.0 - e));
@@ -855,8 +862,8 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
             // Kidder 1995 (general_relativity.rs:177-456)
             const Rcp Rhm{hm, lds.inv_m[hl]};
             const Rcp Rm{m, B.inv_m};
-            const double hmg = lds.mg[hl], pmg = lds.mg[lane];
-            const double f = hmg * pmg / p2(hmg + pmg);
+            const double hmg = mg_of(lds, hl), pmg = mg_of(lds, lane);
+            const double f = lds.kf[lane];
             const double mg = hmg + pmg;
             const double distance_2 = p2(d);
             const Rcp Rdc = rcp(distance_2 * SPEED_OF_LIGHT_2);
@@ -901,7 +908,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
                 const double hI = lds.I[hl], I = lds.I[lane];
                 const double Ls[3] = {hI * hs[0], hI * hs[1], hI * hs[2]};
                 const double M = hm + m;
-                const Rcp RM = rcp(M);
+                const Rcp RM{M, lds.inv_M[lane]};
                 const double mass_factor = divr(hm - m, RM);
                 const double Lp[3] = {I * s[0], I * s[1], I * s[2]};
                 const double n3[3] = {divr(px[0], Rd), divr(px[1], Rd), divr(px[2], Rd)};
@@ -1233,11 +1240,11 @@ __device__ inline void kepler_individual_step(Lane &B, double mass_g, double dt)
 // cumulative G-mass and mass of the host and the bodies before k, k included.
 __device__ inline void jacobi_prefix(const WaveLds &lds, const Lane &B, double *mu, double *eta, int *rank) {
     const int hl = B.hl();
-    double m_g = lds.mg[hl], e = lds.m[hl];
+    double m_g = mg_of(lds, hl), e = lds.m[hl];
     int r = 0;
     for (int j = 0; j < B.n; ++j) {
         if (j == B.host) continue;
-        m_g = m_g + lds.mg[B.gbase + j];
+        m_g = m_g + mg_of(lds, B.gbase + j);
         e = e + lds.m[B.gbase + j];
         if (j == B.k) break;
         ++r;
@@ -1253,9 +1260,9 @@ __device__ inline void kepler_steps(const WaveLds &lds, Lane &B, double dt) {
             int rank;
             jacobi_prefix(lds, B, &mu, &eta, &rank);
         } else if (B.coord == POS_COORD_DEMOCRATIC_HELIOCENTRIC) {
-            mu = lds.mg[B.hl()];
+            mu = mg_of(lds, B.hl());
         } else {
-            mu = lds.mg[B.hl()] + lds.mg[B.lane];
+            mu = mg_of(lds, B.hl()) + mg_of(lds, B.lane);
         }
         kepler_individual_step(B, mu, dt);
     }
