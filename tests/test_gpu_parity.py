@@ -372,3 +372,38 @@ def test_resume_carries_the_pair_dependent_map(Ensemble):
     assert_same_state(out, full, keys=("position", "velocity", "angular_momentum", "spin", "velocity_error",
                                        "angular_momentum_error", "current_iteration", "evolver_left_index",
                                        "pair_dissipation_factor"))
+
+
+def test_ejection_and_close_encounter_freeze_their_systems(Ensemble, oracle):
+    """The panics of universe.rs:224-237 other than the collision: a body
+    beyond 100 AU from the host (ejection) and a body inside the Roche radius
+    (close encounter) set their status bits, freeze only their own systems,
+    and match the oracle bit for bit."""
+    spec = gen.config1(8)
+    spec.system["effects"][[2, 5]] = 0           # point masses
+    host2, b2 = spec.body_offset[2], spec.body_offset[2] + 1
+    spec.body["position"][b2] = spec.body["position"][host2] + np.array([150.0, 0.0, 0.0])
+    host5, b5 = spec.body_offset[5], spec.body_offset[5] + 2
+    spec.body["position"][b5] = spec.body["position"][host5] + np.array([0.002, 0.0, 0.0])
+    ref = oracle.run(spec, 30)
+    with Ensemble(spec) as ens:
+        ens.step(30)
+        out = ens.get_state()
+    assert out["status"][2] & abi.STATUS_EJECTION
+    assert out["status"][5] & abi.STATUS_ROCHE
+    others = np.setdiff1d(np.arange(8), [2, 5])
+    assert np.all(out["current_iteration"][others] == 30)
+    assert np.all(out["current_iteration"][[2, 5]] == 1)
+    assert_same_state(out, ref, where="ejection / close encounter")
+
+
+@pytest.mark.parametrize("group_size", [-1, 0])
+def test_largest_systems_against_oracle(Ensemble, oracle, group_size):
+    """Systems of the maximum size (POS_MAX_BODIES = 16 bodies: four per
+    wavefront) under both lane-group mappings."""
+    spec = gen.config2(24, seed=41, n_min=abi.MAX_BODIES, n_max=abi.MAX_BODIES)
+    ref = oracle.run(spec, 100, threads=8)
+    with Ensemble(spec, mapping="packed", group_size=group_size, steps_per_launch=32) as ens:
+        ens.step(100)
+        out = ens.get_state()
+    assert_same_state(out, ref, where=f"16 bodies G={group_size}")
