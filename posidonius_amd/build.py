@@ -8,7 +8,11 @@
   CPU oracle possible;
 * ``-Xclang -target-feature -Xclang -packed-fp32-ops``: no packed-FP32
   instructions are generated;
-* ``--offload-arch=gfx950``: MI355X only.
+* ``--offload-arch=gfx950``: MI355X only;
+* ``-mllvm -amdgpu-sched-strategy=iterative-maxocc``: the machine scheduler
+  variant that measured fastest for the step kernel (3.15e8 vs 3.01e8
+  system-steps/s with the default strategy on the 65,536-system ensemble).
+  Scheduling reorders instructions only; results are bitwise unchanged.
 """
 from __future__ import annotations
 
@@ -35,6 +39,7 @@ def hipcc() -> str:
 def build_library(verbose: bool = False, extra_flags=None) -> str:
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
            "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-mllvm", "-amdgpu-sched-strategy=iterative-maxocc",
            "-Wall", "-Wno-unused-function",
            "-I", os.path.join(os.path.dirname(HERE), "include"),
            SRC, "-o", OUT + ".tmp"]
