@@ -256,7 +256,8 @@ __global__ __launch_bounds__(64) void effects_kernel(DevEnsemble E, double *out_
     Lane B;
     load_lane<V>(E, lds, B, wave);
     double a[3], dL[3];
-    additional_effects<V>(E, lds, B, false, a, dL);
+    PairGeo unused;
+    additional_effects<V>(E, lds, B, false, a, dL, unused);
     const int nb = E.n_bodies;
     if (B.valid()) {
 #pragma unroll

@@ -651,50 +651,154 @@ __device__ inline void disk_force(const DevEnsemble &E, const WaveLds &lds, cons
     F[2] = factor_migration * pv[2] + factor_eccentricity * sp * px[2] + factor_inclination * pv[2];
 }
 
+// Terms of an orbiting body's pair with its host that depend only on the
+// positions, masses, radii and fixed dissipation parameters.  Within one
+// implicit-midpoint solve the positions do not move, and in the fixed-effect-
+// set kernel (no evolution, no evolving dissipation) the radii and factors do
+// not change either, so these are evaluated once per solve instead of once
+// per iteration.  Each is the same rounded value the in-loop expression gives
+// (same operations in the same order), and a quotient by a hoisted divisor
+// uses its correctly rounded reciprocal (divr), which is exact.
+struct PairGeo {
+    double px[3];            // heliocentric position
+    Rcp Rd;                  // distance and its reciprocal
+    Rcp Rp8;                 // distance^8 and its reciprocal
+    double d5;               // distance^5
+    double os, op;           // tidal orthogonal components (constant_time_lag.rs:206-264)
+    double od, pd, id;       // os / d, op / d, 1 / d
+    double conservative;     // conservative radial tidal component (266-307)
+    double term1, term2;     // the two dissipative radial terms without their velocity factor
+    double m3d5, c15;        // -3 / d^5 and 15 / d^7 (oblate_spheroid.rs:51-60)
+    double mgc, mg_d;        // -(G m_h + G m) / (d^2 c^2) and (G m_h + G m) / d
+    double c1pn, o1pn;       // 1PN: 2 (2 + f) mg / d and (mg / (d^2 c^2)) 2 (2 - f)
+    double c2pn;             // 2PN: 3/4 (12 + 29 f) (mg^2 / d^2)
+    double n3[3];            // unit vector of the separation
+    // carried from one evaluation to the next: the scalar products of the
+    // separation with the host's and the body's spins of the previous
+    // evaluation (tides/common.rs:143-170, rotational_flattening/common.rs:93-120)
+    double rs_star, rs_planet;
+};
+
+// The pair geometry of this lane's body (orbiting lanes; zeros elsewhere),
+// with every lane's position exchanged through rows 0-2, and the scalar
+// products with the spins currently in LDS (those of the last evaluation).
+__device__ inline void pair_geometry(WaveLds &lds, const Lane &B, PairGeo &g) {
+    const int lane = B.lane;
+    const int hl = B.hl();
+    if (B.valid()) { lds.xb[0][lane] = B.x[0]; lds.xb[1][lane] = B.x[1]; lds.xb[2][lane] = B.x[2]; }
+    wsync();
+    double d = 1.;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) g.px[q] = 0.;
+    g.rs_star = 0.;
+    g.rs_planet = 0.;
+    if (B.orbiting()) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) g.px[q] = B.x[q] - lds.xb[q][hl];
+        d = sqrt(p2(g.px[0]) + p2(g.px[1]) + p2(g.px[2]));
+        g.rs_star = g.px[0] * lds.s[0][hl] + g.px[1] * lds.s[1][hl] + g.px[2] * lds.s[2][hl];
+        g.rs_planet = g.px[0] * lds.s[0][lane] + g.px[1] * lds.s[1][lane] + g.px[2] * lds.s[2][lane];
+    }
+    wsync();
+    const double m = B.m, hm = lds.m[hl];
+    const double hR = lds.R[hl], R = lds.R[lane];
+    g.Rd = rcp(d);
+    const double d7 = p7(d);
+    const Rcp Rd7 = rcp(d7);
+    g.d5 = p5(d);
+    g.Rp8 = rcp(p8(d));
+    const double sigma_host = lds.sigma[hl], sigma_pair = lds.sigma[lane];
+    g.os = divr(4.5 * p2(m) * p10(hR) * sigma_host, Rd7);
+    g.op = divr(4.5 * p2(hm) * p10(R) * sigma_pair, Rd7);
+    const double hm2 = hm * hm;
+    const double pm2 = m * m;
+    g.conservative = divr(-3.0 * K2, Rd7) * (pm2 * p5(hR) * lds.k2[hl] + hm2 * p5(R) * lds.k2[lane]);
+    g.term1 = pm2 * p10(hR) * sigma_host;
+    g.term2 = hm2 * p10(R) * sigma_pair;
+    g.od = divr(g.os, g.Rd);
+    g.pd = divr(g.op, g.Rd);
+    g.id = divr(1.0, g.Rd);
+    g.m3d5 = -3. / g.d5;
+    g.c15 = divr(15., Rd7);
+    const double hmg = mg_of(lds, hl), pmg = mg_of(lds, lane);
+    const double f = lds.kf[lane];
+    const double mg = hmg + pmg;
+    const double distance_2 = p2(d);
+    const Rcp Rdc = rcp(distance_2 * SPEED_OF_LIGHT_2);
+    g.mgc = divr(-mg, Rdc);
+    g.mg_d = divr(mg, g.Rd);
+    g.c1pn = divr(2.0 * (2.0 + f) * mg, g.Rd);
+    g.o1pn = divr(mg, Rdc) * 2.0 * (2.0 - f);
+    g.c2pn = 3.0 / 4.0 * (12.0 + 29.0 * f) * (p2(mg) / distance_2);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) g.n3[q] = divr(g.px[q], g.Rd);
+}
+
 // Accelerations and spin derivatives of every body of the group at the current
 // (averaged) state.  Outputs are accumulated in the reference's order:
 // a_add = ((0 + tides) + flattening) + GR and
 // dL/dt = ((tides + flattening) + GR) + wind (universe.rs:540-614).
-template <int V>
+// With H (the fixed-effect-set kernel inside an implicit-midpoint solve) the
+// pair terms come from geo, which pair_geometry filled at the start of the
+// solve, and the spins' scalar products are carried in geo from one
+// evaluation to the next; without H geo is not read.
+template <int V, bool H = false>
 __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, Lane &B, bool evolution,
-                                          double a_add[3], double dLdt[3]) {
+                                          double a_add[3], double dLdt[3], PairGeo &geo) {
     constexpr int GRV = V & 3;
     constexpr bool EXT = (V & VAR_EXT) != 0;
+    static_assert(!H || ((V & VAR_FAST) != 0 && !EXT), "hoisted pair terms need fixed radii and factors");
     const int lane = B.lane;
     const int hl = B.hl();
     const bool orbiting = B.orbiting();
     // ---- inertial -> heliocentric (universe.rs:318-351); the host's
     //      heliocentric velocity is zero at this point ----
     if (B.valid()) {
-        lds.xb[0][lane] = B.x[0]; lds.xb[1][lane] = B.x[1]; lds.xb[2][lane] = B.x[2];
+        if (!H) { lds.xb[0][lane] = B.x[0]; lds.xb[1][lane] = B.x[1]; lds.xb[2][lane] = B.x[2]; }
         lds.xb[3][lane] = B.v[0]; lds.xb[4][lane] = B.v[1]; lds.xb[5][lane] = B.v[2];
     }
+    // With H the old spins are not read from LDS (their scalar products are
+    // carried), so the new spins are published in the same exchange as the
+    // velocities.
+    if (H) own_spin(lds, B);
     wsync();
     double px[3] = {0., 0., 0.}, pv[3] = {0., 0., 0.};
     double d = 1., nv2 = 0., nv = 1.;
     double rs_star = 0., rs_planet = 0.;
+    if (H) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) px[q] = geo.px[q];
+        d = geo.Rd.b;
+    }
     if (orbiting) {
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            px[q] = B.x[q] - lds.xb[q][hl];
+            if (!H) px[q] = B.x[q] - lds.xb[q][hl];
             pv[q] = B.v[q] - lds.xb[3 + q][hl];
         }
-        d = sqrt(p2(px[0]) + p2(px[1]) + p2(px[2]));
+        if (!H) d = sqrt(p2(px[0]) + p2(px[1]) + p2(px[2]));
         nv2 = p2(pv[0]) + p2(pv[1]) + p2(pv[2]);
         nv = sqrt(nv2);
         // Universe::initialize runs before the spins are refreshed
         // (universe.rs:429-430), so the scalar products of tides/common.rs:143-170
         // and rotational_flattening/common.rs:93-120 use the spins of the
         // previous evaluation.
-        rs_star = px[0] * lds.s[0][hl] + px[1] * lds.s[1][hl] + px[2] * lds.s[2][hl];
-        rs_planet = px[0] * lds.s[0][lane] + px[1] * lds.s[1][lane] + px[2] * lds.s[2][lane];
+        if (H) {
+            rs_star = geo.rs_star;
+            rs_planet = geo.rs_planet;
+        } else {
+            rs_star = px[0] * lds.s[0][hl] + px[1] * lds.s[1][hl] + px[2] * lds.s[2][hl];
+            rs_planet = px[0] * lds.s[0][lane] + px[1] * lds.s[1][lane] + px[2] * lds.s[2][lane];
+        }
     }
-    const Rcp Rd = rcp(d);
+    const Rcp Rd = H ? geo.Rd : rcp(d);
     const double vr = orbiting ? divr(px[0] * pv[0] + px[1] * pv[1] + px[2] * pv[2], Rd) : 0.;
-    wsync();
-    // ---- evolution and spin = L / I (universe.rs:305-316) ----
-    if (evolution && (B.flags & F_EVOLUTION)) evolve_body(E, lds, B, B.t);
-    own_spin(lds, B);
+    if (!H) {
+        wsync();
+        // ---- evolution and spin = L / I (universe.rs:305-316) ----
+        if (evolution && (B.flags & F_EVOLUTION)) evolve_body(E, lds, B, B.t);
+        own_spin(lds, B);
+    }
     double dk[3] = {0., 0., 0.};   // disk acceleration of this body
     double host_lag = 0.;
     if (EXT) {
@@ -712,7 +816,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
             lds.xb[3][lane] = B.is_host() ? E.lag[B.body] : ((B.flags & F_DISK_ROLE) ? 1. : 0.);
         }
     }
-    wsync();
+    if (!H) wsync();
     if (EXT) {
         if (B.valid() && B.is_host() && (B.flags & F_DISK)) {   // disk.rs:259-265, bodies in index order
             double sum[3] = {0., 0., 0.};
@@ -738,9 +842,9 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         const double s[3] = {lds.s[0][lane], lds.s[1][lane], lds.s[2][lane]};
         const double hs2 = lds.s2[hl], s2 = lds.s2[lane];
         const double hR = lds.R[hl], R = lds.R[lane];
-        const double d7 = p7(d);
-        const Rcp Rd7 = rcp(d7);
-        const double d5 = p5(d);
+        const double d7 = H ? 0. : p7(d);
+        const Rcp Rd7 = H ? Rcp{0., 0.} : rcp(d7);
+        const double d5 = H ? geo.d5 : p5(d);
         // ---- the dissipation factors of this host-body pair: the bodies' own
         //      scaled factors, or for evolving-dissipation models the pair
         //      values of a dynamical tide this orbit excites
@@ -775,7 +879,10 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         }
         // ---- tides: orthogonal components (constant_time_lag.rs:206-264) ----
         double os = 0., op = 0.;
-        if (B.tides()) {
+        if (H) {
+            os = geo.os;
+            op = geo.op;
+        } else if (B.tides()) {
             os = divr(4.5 * p2(m) * p10(hR) * sigma_host, Rd7);
             op = divr(4.5 * p2(hm) * p10(R) * sigma_pair, Rd7);
         }
@@ -792,16 +899,17 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
             // radial component (constant_time_lag.rs:266-307)
             const double hm2 = hm * hm;
             const double pm2 = m * m;
-            const double conservative = divr(-3.0 * K2, Rd7) * (pm2 * p5(hR) * lds.k2[hl] + hm2 * p5(R) * lds.k2[lane]);
-            const double factor1 = -13.5 * vr / p8(d);
-            const double term1 = pm2 * p10(hR) * sigma_host;
-            const double term2 = hm2 * p10(R) * sigma_pair;
+            const double conservative = H ? geo.conservative
+                                          : divr(-3.0 * K2, Rd7) * (pm2 * p5(hR) * lds.k2[hl] + hm2 * p5(R) * lds.k2[lane]);
+            const double factor1 = H ? divr(-13.5 * vr, geo.Rp8) : -13.5 * vr / p8(d);
+            const double term1 = H ? geo.term1 : pm2 * p10(hR) * sigma_host;
+            const double term2 = H ? geo.term2 : hm2 * p10(R) * sigma_pair;
             const double diss_pm = factor1 * term2;
             const double dissipative = diss_pm + factor1 * term1;
             const double radial = conservative + dissipative;
             // force (constant_time_lag.rs:309-332)
             const double factor3 = radial + divr((os + op) * vr, Rd);
-            const double od = divr(os, Rd), pd = divr(op, Rd);
+            const double od = H ? geo.od : divr(os, Rd), pd = H ? geo.pd : divr(op, Rd);
             const double F0 = divr(factor3 * px[0], Rd) + od * (hs[1] * px[2] - hs[2] * px[1] - pv[0])
                             + pd * (s[1] * px[2] - s[2] * px[1] - pv[0]);
             const double F1 = divr(factor3 * px[1], Rd) + od * (hs[2] * px[0] - hs[0] * px[2] - pv[1])
@@ -819,8 +927,8 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         }
         if (B.rf()) {
             // radial component (oblate_spheroid.rs:51-60) and force (83-97)
-            const double radial = -3. / d5 * (fp + fs)
-                + divr(15., Rd7) * (fs * rs_star * rs_star / hs2 + fp * rs_planet * rs_planet / s2);
+            const double radial = (H ? geo.m3d5 : -3. / d5) * (fp + fs)
+                + (H ? geo.c15 : divr(15., Rd7)) * (fs * rs_star * rs_star / hs2 + fp * rs_planet * rs_planet / s2);
             const double F0 = radial * px[0] + ofp * s[0] + ofs * hs[0];
             const double F1 = radial * px[1] + ofp * s[1] + ofs * hs[1];
             const double F2 = radial * px[2] + ofp * s[2] + ofs * hs[2];
@@ -837,7 +945,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
                 const double cx = px[1] * pv[2] - px[2] * pv[1];
                 const double cy = px[2] * pv[0] - px[0] * pv[2];
                 const double cz = px[0] * pv[1] - px[1] * pv[0];
-                const double id = divr(1.0, Rd);
+                const double id = H ? geo.id : divr(1.0, Rd);
                 tdL[0] = -1.0 * (op * (d * s[0] - divr(rs_planet * px[0], Rd) - id * cx));
                 tdL[1] = -1.0 * (op * (d * s[1] - divr(rs_planet * px[1], Rd) - id * cy));
                 tdL[2] = -1.0 * (op * (d * s[2] - divr(rs_planet * px[2], Rd) - id * cz));
@@ -865,17 +973,17 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
             const double hmg = mg_of(lds, hl), pmg = mg_of(lds, lane);
             const double f = lds.kf[lane];
             const double mg = hmg + pmg;
-            const double distance_2 = p2(d);
-            const Rcp Rdc = rcp(distance_2 * SPEED_OF_LIGHT_2);
-            const double mgc = divr(-mg, Rdc);          // -mg / (d^2 c^2)
-            const double mg_d = divr(mg, Rd);
+            const double distance_2 = H ? 0. : p2(d);
+            const Rcp Rdc = H ? Rcp{0., 0.} : rcp(distance_2 * SPEED_OF_LIGHT_2);
+            const double mgc = H ? geo.mgc : divr(-mg, Rdc);          // -mg / (d^2 c^2)
+            const double mg_d = H ? geo.mg_d : divr(mg, Rd);
             const double mOverM = divr(m, Rhm);
             const Rcp Rnv = rcp(nv);
             double gr[3];
             {   // 1PN
                 const double radial_velocity_2 = p2(vr);
-                const double radial = mgc * ((1.0 + 3.0 * f) * nv2 - divr(2.0 * (2.0 + f) * mg, Rd) - 1.5 * f * radial_velocity_2);
-                const double orthogonal = divr(mg, Rdc) * 2.0 * (2.0 - f) * vr * nv;
+                const double radial = mgc * ((1.0 + 3.0 * f) * nv2 - (H ? geo.c1pn : divr(2.0 * (2.0 + f) * mg, Rd)) - 1.5 * f * radial_velocity_2);
+                const double orthogonal = (H ? geo.o1pn : divr(mg, Rdc) * 2.0 * (2.0 - f)) * vr * nv;
 #pragma unroll
                 for (int q = 0; q < 3; ++q) gr[q] = divr(radial * px[q], Rd) + divr(orthogonal * pv[q], Rnv);
                 put3(lds, C_G1, lane, mOverM * gr[0], mOverM * gr[1], mOverM * gr[2]);
@@ -886,7 +994,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
                 const double vr4 = p2(vr2);
                 const double f2 = p2(f);
                 const double radial = mgc
-                    * (3.0 / 4.0 * (12.0 + 29.0 * f) * (p2(mg) / distance_2)
+                    * ((H ? geo.c2pn : 3.0 / 4.0 * (12.0 + 29.0 * f) * (p2(mg) / distance_2))
                        + f * (3.0 - 4.0 * f) * nv4
                        + 15.0 / 8.0 * f * (1.0 - 3.0 * f) * vr4
                        - 3.0 / 2.0 * f * (3.0 - 4.0 * f) * vr2 * nv2
@@ -911,7 +1019,8 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
                 const Rcp RM{M, lds.inv_M[lane]};
                 const double mass_factor = divr(hm - m, RM);
                 const double Lp[3] = {I * s[0], I * s[1], I * s[2]};
-                const double n3[3] = {divr(px[0], Rd), divr(px[1], Rd), divr(px[2], Rd)};
+                const double n3[3] = {H ? geo.n3[0] : divr(px[0], Rd), H ? geo.n3[1] : divr(px[1], Rd),
+                                      H ? geo.n3[2] : divr(px[2], Rd)};
                 const double factor_a = G / SPEED_OF_LIGHT_2;
                 double a[3];
                 {
@@ -952,6 +1061,10 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         } else {
 #pragma unroll
             for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + 0.;
+        }
+        if (H) {   // for the next evaluation, whose spins are refreshed before it reads them
+            geo.rs_star = px[0] * hs[0] + px[1] * hs[1] + px[2] * hs[2];
+            geo.rs_planet = px[0] * s[0] + px[1] * s[1] + px[2] * s[2];
         }
     }
     wsync();
@@ -1027,63 +1140,69 @@ template <int V>
 __device__ inline void integrate_velocity_dependent_forces(const DevEnsemble &E, WaveLds &lds, Lane &B, bool evolution) {
     const double h = 0.5 * B.dt;
     const bool spin_on = B.spin_on();
-    double v_orig[3], L_orig[3], v_fin[3], L_fin[3], dv[3] = {0., 0., 0.}, dL[3] = {0., 0., 0.};
+    // The final state of an iteration is v_orig + dv (and L_orig + dL), so only
+    // the increments are carried from one iteration to the next; the previous
+    // final state and the midpoint are recomputed from them, bit for bit.
+    double v_orig[3], L_orig[3], dv[3] = {0., 0., 0.}, dL[3] = {0., 0., 0.};
 #pragma unroll
-    for (int q = 0; q < 3; ++q) { v_orig[q] = B.v[q]; L_orig[q] = B.L[q]; v_fin[q] = B.v[q]; L_fin[q] = B.L[q]; }
+    for (int q = 0; q < 3; ++q) { v_orig[q] = B.v[q]; L_orig[q] = B.L[q]; }
+    constexpr bool H = (V & VAR_FAST) != 0;   // fixed radii and factors: hoist the pair terms
+    PairGeo geo;
+    if (H) pair_geometry(lds, B, geo);
     bool converged = false;
     for (int it = 0; it < MIDPOINT_MAX_ITER; ++it) {
         double a_add[3], dLdt[3];
-        additional_effects<V>(E, lds, B, evolution && it == 0, a_add, dLdt);
+        additional_effects<V, H>(E, lds, B, evolution && it == 0, a_add, dLdt, geo);
         double pdv = 0., pdL = 0.;   // |change of the final state|^2 of this body
+        double fv2 = 0., fL2 = 0.;   // |final state|^2 of this body
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
+            const double v_prev = it == 0 ? v_orig[q] : v_orig[q] + dv[q];
             dv[q] = h * a_add[q] - B.ev[q];
             const double vf = v_orig[q] + dv[q];
-            pdv = pdv + p2(vf - v_fin[q]);
-            v_fin[q] = vf;
+            pdv = pdv + p2(vf - v_prev);
+            fv2 = fv2 + p2(vf);
+            B.v[q] = 0.5 * (v_orig[q] + vf);
             if (spin_on) {
+                const double L_prev = it == 0 ? L_orig[q] : L_orig[q] + dL[q];
                 dL[q] = h * dLdt[q] - B.eL[q];
                 const double Lf = L_orig[q] + dL[q];
-                pdL = pdL + p2(Lf - L_fin[q]);
-                L_fin[q] = Lf;
+                pdL = pdL + p2(Lf - L_prev);
+                fL2 = fL2 + p2(Lf);
+                B.L[q] = 0.5 * (L_orig[q] + Lf);
             }
         }
         if (it >= MIDPOINT_MIN_ITER - 1) {
-            // whfast.rs:424-451, sums over bodies in body order
+            // whfast.rs:424-451: four sums over the bodies in body order, which
+            // every lane of the group forms itself from the exchange rows
             if (B.valid()) {
                 lds.xb[0][B.lane] = pdv;
-                lds.xb[1][B.lane] = p2(v_fin[0]) + p2(v_fin[1]) + p2(v_fin[2]);
+                lds.xb[1][B.lane] = fv2;
                 lds.xb[2][B.lane] = pdL;
-                lds.xb[3][B.lane] = p2(L_fin[0]) + p2(L_fin[1]) + p2(L_fin[2]);
+                lds.xb[3][B.lane] = fL2;
             }
             wsync();
-            // four ordered sums over all bodies, spread over the lanes, into rows 4-7
-            for (int q = B.k; B.valid() && q < 4; q += B.n) {
-                double acc = 0.;
-                for (int j = 0; j < B.n; ++j) acc = acc + lds.xb[q][B.gbase + j];
-                lds.xb[4 + q][B.gbase] = acc;
+            double dv2 = 0., sv2 = 0., dL2 = 0., sL2 = 0.;
+            for (int j = 0; j < B.n; ++j) {
+                const int l = B.gbase + j;
+                dv2 = dv2 + lds.xb[0][l];
+                sv2 = sv2 + lds.xb[1][l];
+                dL2 = dL2 + lds.xb[2][l];
+                sL2 = sL2 + lds.xb[3][l];
             }
             wsync();
-            const double dv2 = lds.xb[4][B.gbase], fv2 = lds.xb[5][B.gbase];
-            const double dL2 = lds.xb[6][B.gbase], fL2 = lds.xb[7][B.gbase];
-            wsync();
-            const bool ok = (spin_on && dv2 / fv2 < DBL_EPS_2 && dL2 / fL2 < DBL_EPS_2) ||
-                            (!spin_on && dv2 / fv2 < DBL_EPS_2);
+            const bool ok = (spin_on && dv2 / sv2 < DBL_EPS_2 && dL2 / sL2 < DBL_EPS_2) ||
+                            (!spin_on && dv2 / sv2 < DBL_EPS_2);
             if (ok) { converged = true; break; }
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            B.v[q] = 0.5 * (v_orig[q] + v_fin[q]);
-            if (spin_on) B.L[q] = 0.5 * (L_orig[q] + L_fin[q]);
         }
     }
     if (!converged) B.status |= POS_STATUS_MIDPOINT_NOT_CONVERGED;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        B.v[q] = v_fin[q];
+        B.v[q] = v_orig[q] + dv[q];
         B.ev[q] = (B.v[q] - v_orig[q]) - dv[q];
         if (spin_on) {
-            B.L[q] = L_fin[q];
+            B.L[q] = L_orig[q] + dL[q];
             B.eL[q] = (B.L[q] - L_orig[q]) - dL[q];
         }
     }
