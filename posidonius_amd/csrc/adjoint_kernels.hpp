@@ -27,6 +27,13 @@
 
 #include "whfast_step.hpp"
 
+// The adjoint is compared with finite differences and the tangent-linear run,
+// not bit for bit with anything, so its arithmetic (and only its: the forward
+// step above keeps every multiply and add separately rounded) may fuse
+// multiplies and adds.  The pragma covers every function defined from here to
+// the matching contract(off) at the end of this header.
+#pragma clang fp contract(fast)
+
 #define POS_HD __device__ inline
 #include "adjoint_math.hpp"
 
@@ -414,3 +421,5 @@ __device__ inline void adjoint_store(const AdjArgs &Ar, const AdjLane &A) {
 }
 
 }  // namespace posdev
+
+#pragma clang fp contract(off)

@@ -152,6 +152,12 @@ POS_HD void pair_vjp(const PairConst &C, const PairPar &P, const PairIn &I, cons
     const double vr = ru * id;
     const double rsh = dot(r, I.Wh), rsp = dot(r, I.Wp);
     const double d5 = d2 * d2 * d, d7 = d5 * d2, d8 = d7 * d;
+    // Reciprocals of the linearisation point's distances: every quotient of
+    // the reverse part below is a cotangent times a coefficient that depends
+    // on the linearisation point only, so a caller that repeats the product
+    // at a fixed point (the implicit-midpoint fixed point) evaluates the
+    // coefficients once.
+    const double id5 = 1. / d5, id7 = 1. / d7, id8 = 1. / d8;
     // adjoints of the shared intermediates
     double d_b = 0., d2_b = 0., id_b = 0., u2_b = 0., vr_b = 0., rsh_b = 0., rsp_b = 0.;
     double d5_b = 0., d7_b = 0., d8_b = 0.;
@@ -205,13 +211,13 @@ POS_HD void pair_vjp(const PairConst &C, const PairPar &P, const PairIn &I, cons
         cross_adj(r, u, c_b, r_b, u_b);
         // f3 = -3 K2 Ck / d7 - 2 vr SA / d8
         const double Ck_b = f3_b * (-3. * G_K2 / d7);
-        d7_b += f3_b * 3. * G_K2 * Ck / (d7 * d7);
+        d7_b += f3_b * (3. * G_K2 * Ck * id7 * id7);
         vr_b += f3_b * (-2. * SA / d8);
         const double SA_b = f3_b * (-2. * vr / d8);
-        d8_b += f3_b * 2. * vr * SA / (d8 * d8);
+        d8_b += f3_b * (2. * vr * SA * id8 * id8);
         // os = Ah / d7, op = Ap / d7
-        const double Ah_b = os_b / d7 + SA_b, Ap_b = op_b / d7 + SA_b;
-        d7_b += -os_b * Ah / (d7 * d7) - op_b * Ap / (d7 * d7);
+        const double Ah_b = os_b * id7 + SA_b, Ap_b = op_b * id7 + SA_b;
+        d7_b += -(os_b * (Ah * id7 * id7)) - op_b * (Ap * id7 * id7);
         Pb.sh += Ah_b * Ah_c;
         Pb.sp += Ap_b * Ap_c;
         Pb.k2h += Ck_b * Kh;
@@ -237,19 +243,19 @@ POS_HD void pair_vjp(const PairConst &C, const PairPar &P, const PairIn &I, cons
         cross_adj(r, I.Wh, (-ofs) * ob.Nh, r_b, Wh_b);
         // radf
         const double Wp2_b = radf_b * (-3. * Bp / d5), Wh2_b = radf_b * (-3. * Bh / d5);
-        d5_b += radf_b * 3. * (Bp * Wp2 + Bh * Wh2) / (d5 * d5);
-        rsh_b += radf_b * 30. * Bh * rsh / d7;
-        rsp_b += radf_b * 30. * Bp * rsp / d7;
-        d7_b += -radf_b * 15. * (Bh * rsh * rsh + Bp * rsp * rsp) / (d7 * d7);
+        d5_b += radf_b * (3. * (Bp * Wp2 + Bh * Wh2) * id5 * id5);
+        rsh_b += radf_b * (30. * Bh * rsh * id7);
+        rsp_b += radf_b * (30. * Bp * rsp * id7);
+        d7_b += -radf_b * (15. * (Bh * rsh * rsh + Bp * rsp * rsp) * id7 * id7);
         double Bp_b = radf_b * (-3. * Wp2 / d5 + 15. * rsp * rsp / d7);
         double Bh_b = radf_b * (-3. * Wh2 / d5 + 15. * rsh * rsh / d7);
         // ofs, ofp
         Bh_b += ofs_b * (-6. * rsh / d5);
         rsh_b += ofs_b * (-6. * Bh / d5);
-        d5_b += ofs_b * 6. * Bh * rsh / (d5 * d5);
+        d5_b += ofs_b * (6. * Bh * rsh * id5 * id5);
         Bp_b += ofp_b * (-6. * rsp / d5);
         rsp_b += ofp_b * (-6. * Bp / d5);
-        d5_b += ofp_b * 6. * Bp * rsp / (d5 * d5);
+        d5_b += ofp_b * (6. * Bp * rsp * id5 * id5);
         Wh_b += (2. * Wh2_b) * I.Wh;
         Wp_b += (2. * Wp2_b) * I.Wp;
         Pb.kfh += Bh_b * Bh_c;
@@ -350,8 +356,8 @@ POS_HD void pair_vjp(const PairConst &C, const PairPar &P, const PairIn &I, cons
         r_b += id * n_b;
         // shared GR scalars
         vr_b += 2. * vr * vr2_b;
-        d_b += -mgd_b * mg / (d * d);
-        d2_b += mgc_b * mg / (d2 * d2 * C2);
+        d_b += -mgd_b * (mg * id * id);
+        d2_b += mgc_b * (mg / (d2 * d2 * C2));
     }
     // shared intermediates
     d_b += 5. * d5 / d * d5_b + 7. * d7 / d * d7_b + 8. * d8 / d * d8_b;
@@ -365,8 +371,8 @@ POS_HD void pair_vjp(const PairConst &C, const PairPar &P, const PairIn &I, cons
     Wh_b += rsh_b * r;
     r_b += rsp_b * I.Wp;
     Wp_b += rsp_b * r;
-    d_b += -id_b / d2;
-    d2_b += d_b * 0.5 / d;
+    d_b += -id_b * (id * id);
+    d2_b += d_b * (0.5 * id);
     r_b += (2. * d2_b) * r;
     Ib.r += r_b;
     Ib.u += u_b;
