@@ -109,7 +109,10 @@ __device__ __forceinline__ V3 get3(const AdjLds &L, int row, int lane) {
 // One pass of the effects' vector-Jacobian product at the linearisation
 // point published in rows 0-8 (x, v, W = L / I of every body): given the
 // cotangents (mu_a, mu_L) of every body's additional acceleration and dL/dt,
-// return this body's cotangents of x, v, L and of its parameters.
+// return this body's cotangents of v and L and, with FULL, of x and of its
+// parameters (without FULL those are zero and the work only they need is
+// dropped by the compiler).
+template <bool FULL>
 __device__ inline void effects_vjp_pass(AdjLds &L, const AdjLane &A, const posadj::PairConst &C,
                                         const posadj::PairPar &P, const posadj::PairIn &in, V3 mu_a, V3 mu_L,
                                         V3 &gx, V3 &gv, V3 &gL, double pg[3]) {
@@ -129,22 +132,26 @@ __device__ inline void effects_vjp_pass(AdjLds &L, const AdjLane &A, const posad
         posadj::PairIn Ib{V3{0., 0., 0.}, V3{0., 0., 0.}, V3{0., 0., 0.}, V3{0., 0., 0.}};
         posadj::PairParBar Pb{0., 0., 0., 0., 0., 0.};
         posadj::pair_vjp(C, P, in, ob, Ib, Pb);
-        gx = Ib.r;
         gv = Ib.u;
         gL = (1. / A.I) * Ib.Wp;
-        pg[0] = Pb.sp; pg[1] = Pb.k2p; pg[2] = Pb.kfp;
-        put3(L, 15, lane, -Ib.r);
         put3(L, 18, lane, -Ib.u);
         put3(L, 21, lane, (1. / A.Ih) * Ib.Wh);
-        put3(L, 24, lane, V3{Pb.sh, Pb.k2h, Pb.kfh});
+        if (FULL) {
+            gx = Ib.r;
+            pg[0] = Pb.sp; pg[1] = Pb.k2p; pg[2] = Pb.kfp;
+            put3(L, 15, lane, -Ib.r);
+            put3(L, 24, lane, V3{Pb.sh, Pb.k2h, Pb.kfh});
+        }
     }
     wsync();
     if (A.valid && A.is_host) {
-        gx = gsum3(L, A, 15, true);
         gv = gsum3(L, A, 18, true);
         gL = gsum3(L, A, 21, true);
-        const V3 p = gsum3(L, A, 24, true);
-        pg[0] = p.x; pg[1] = p.y; pg[2] = p.z;
+        if (FULL) {
+            gx = gsum3(L, A, 15, true);
+            const V3 p = gsum3(L, A, 24, true);
+            pg[0] = p.x; pg[1] = p.y; pg[2] = p.z;
+        }
     }
     wsync();
 }
@@ -190,10 +197,11 @@ __device__ inline void kick_adjoint(AdjLds &L, AdjLane &A, V3 x, V3 v0, V3 L0, V
     const V3 z{0., 0., 0.};
     V3 gx, gv, gL;
     double pg[3];
-    // fixed point of mu = lambda + (h/2) F_y^T mu, starting from mu = lambda
+    // fixed point of mu = lambda + (h/2) F_y^T mu, starting from mu = lambda;
+    // the passes need only the v and L parts of the product
     V3 mu_v = lam_v, mu_L = A.spin ? lam_L : z;
     for (int pass = 0;; ++pass) {
-        effects_vjp_pass(L, A, C, P, in, mu_v, mu_L, gx, gv, gL, pg);
+        effects_vjp_pass<false>(L, A, C, P, in, mu_v, mu_L, gx, gv, gL, pg);
         const V3 nv = lam_v + (0.5 * h) * gv;
         const V3 nL = A.spin ? lam_L + (0.5 * h) * gL : z;
         const bool conv = !A.valid || (settled(nv, mu_v) && settled(nL, mu_L));
@@ -201,6 +209,8 @@ __device__ inline void kick_adjoint(AdjLds &L, AdjLane &A, V3 x, V3 v0, V3 L0, V
         mu_L = nL;
         if (pass + 1 >= ADJ_MAX_PASSES || group_all(L, A, conv)) break;
     }
+    // the x and parameter parts (and, without spin, the L part) at the fixed point
+    effects_vjp_pass<true>(L, A, C, P, in, mu_v, mu_L, gx, gv, gL, pg);
     // y0's cotangent is mu + (h/2) F_y^T mu = 2 mu - lambda
     st3(A.lv, 2. * mu_v - lam_v);
     st3(A.lL, A.spin ? 2. * mu_L - lam_L : lam_L + h * gL);

@@ -181,17 +181,20 @@ void whfast_steps_kernel(DevEnsemble E, int n_steps, int wave_begin, int wave_en
     }
 }
 
-// The same K steps, also writing the step tape of the adjoint (one row per
-// step plus the final state).
+// The same K steps over waves [wave_begin, wave_end) of variant V (the
+// adjoint admits the generic Kidder variant and the fixed-effect-set one),
+// also writing the step tape of the adjoint (one row per step plus the final
+// state).
+template <int V>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POS_WAVES_PER_EU)))
-void whfast_record_kernel(DevEnsemble E, int n_steps, TapeRef T) {
+void whfast_record_kernel(DevEnsemble E, int n_steps, TapeRef T, int wave_begin, int wave_end) {
     __shared__ WaveLds lds;
-    for (int wave = blockIdx.x; wave < E.n_waves; wave += gridDim.x) {
+    for (int wave = wave_begin + blockIdx.x; wave < wave_end; wave += gridDim.x) {
         Lane B;
-        load_lane(E, lds, B, wave);
+        load_lane<V>(E, lds, B, wave);
         for (int step = 0; step < n_steps; ++step) {
             if (B.status & FROZEN) break;   // uniform within the group
-            iterate<true, GRV_KIDDER>(E, lds, B, T, step);   // the adjoint excludes the other GR variants
+            iterate<true, V>(E, lds, B, T, step);
         }
         store_lane(E, lds, B);
         __syncthreads();
@@ -1470,9 +1473,20 @@ int32_t pos_adjoint(pos_ens *h, const pos_adjoint_seed *seed, pos_adjoint_grad *
         if (s + 2 < A.n_seg) HIP_TRY(hipStreamWaitEvent(h->stream, A.swept_ev[buf], 0));
         int r = adjoint_copy_state(h, A.ckpt + (size_t)s * A.state_bytes, false);
         if (r) return r;
-        hipLaunchKernelGGL(whfast_record_kernel, dim3(h->grid), dim3(64), 0, h->stream, h->E, seg_len(s),
-                           TapeRef{A.tape[buf], nb});
-        HIP_TRY(hipGetLastError());
+        for (int v = 0; v < N_VARIANTS; ++v) {   // pos_adjoint_record admits variants 0 and VAR_FAST only
+            const int w0 = h->variant_waves[v][0], w1 = h->variant_waves[v][1];
+            if (w1 <= w0) continue;
+            const dim3 grid(std::min(h->grid, w1 - w0));
+            const TapeRef tape{A.tape[buf], nb};
+            if (v == GRV_KIDDER)
+                hipLaunchKernelGGL(whfast_record_kernel<GRV_KIDDER>, grid, dim3(64), 0, h->stream, h->E, seg_len(s), tape, w0, w1);
+            else if (v == (GRV_KIDDER | VAR_FAST))
+                hipLaunchKernelGGL(whfast_record_kernel<GRV_KIDDER | VAR_FAST>, grid, dim3(64), 0, h->stream, h->E, seg_len(s), tape,
+                                   w0, w1);
+            else
+                return fail(POS_E_INVALID, "internal error: kernel variant without an adjoint");
+            HIP_TRY(hipGetLastError());
+        }
         HIP_TRY(hipEventRecord(A.recorded_ev[buf], h->stream));
         return POS_OK;
     };
