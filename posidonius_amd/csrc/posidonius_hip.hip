@@ -634,6 +634,22 @@ int launch_steps(pos_ens *h, int n_steps) {
     return POS_OK;
 }
 
+// GRAPH_LAUNCHES chunk launches of the current launch table, captured from
+// the handle's stream (nothing executes) and instantiated once; rebuilt after
+// anything that changes the kernels' arguments.
+int ensure_graph(pos_ens *h) {
+    if (h->graph_exec) return POS_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));   // capture starts on an idle stream
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int crc = POS_OK;
+    for (int i = 0; i < GRAPH_LAUNCHES && crc == POS_OK; ++i) crc = launch_steps(h, h->steps_per_launch);
+    HIP_TRY(hipStreamEndCapture(h->stream, &h->graph));
+    if (crc) return crc;
+    HIP_TRY(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+    h->graph_launches = GRAPH_LAUNCHES;
+    return POS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -855,6 +871,8 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     h->E.hist = nullptr;
     h->E.hist_count = nullptr;
     h->E.hist_cap = 0;
+    // with graph replay, capture now rather than inside the first long pos_step
+    if (h->opts.use_graph && (rc = ensure_graph(h))) { pos_destroy(h); return rc; }
     *out = h;
     return POS_OK;
 }
@@ -867,16 +885,8 @@ int32_t pos_step(pos_ens *h, int64_t n_steps) {
     int64_t full = n_steps / K;
     const int rest = (int)(n_steps - full * K);
     if (h->opts.use_graph && full >= GRAPH_LAUNCHES) {
-        // GRAPH_LAUNCHES chunk launches captured once per launch table, replayed
-        if (!h->graph_exec) {
-            HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            int crc = POS_OK;
-            for (int i = 0; i < GRAPH_LAUNCHES && crc == POS_OK; ++i) crc = launch_steps(h, K);
-            HIP_TRY(hipStreamEndCapture(h->stream, &h->graph));
-            if (crc) return crc;
-            HIP_TRY(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
-            h->graph_launches = GRAPH_LAUNCHES;
-        }
+        int grc = ensure_graph(h);
+        if (grc) return grc;
         while (full >= GRAPH_LAUNCHES) {
             HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
             full -= GRAPH_LAUNCHES;
@@ -1234,9 +1244,9 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
         }
     }
     // graph replay against direct launches for the chosen mapping: a graph
-    // saves launch overhead on short launches, but its kernel nodes have
-    // measured slower than direct launches of long ones
-    best.use_graph = 0;
+    // saves host work and launch gaps on short launches and measures the same
+    // as direct launches on long ones, so it is kept unless direct launches
+    // are faster by more than the timing spread (2%)
     if (keep_graph) {
         double t_mode[2] = {0., 0.};
         for (int mode = 0; mode < 2; ++mode) {
@@ -1254,11 +1264,14 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
             t_mode[mode] = ms;
             if ((rc = restore())) return rc;
         }
-        best.use_graph = t_mode[1] < t_mode[0] ? 1 : 0;
+        best.use_graph = t_mode[1] <= 1.02 * t_mode[0] ? 1 : 0;
+    } else {
+        best.use_graph = 0;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if ((rc = build_launch(h, best))) return rc;
+    if (best.use_graph && (rc = ensure_graph(h))) return rc;   // captured now, not in the caller's first steps
     if (chosen) *chosen = h->opts;
     return POS_OK;
 }
@@ -1286,6 +1299,7 @@ int32_t pos_history_enable(pos_ens *h, int32_t capacity) {
     if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
     if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
     h->graph_launches = 0;
+    if (h->opts.use_graph && (rc = ensure_graph(h))) return rc;
     return POS_OK;
 }
 
