@@ -143,10 +143,33 @@ def cpu_baseline(spec, threads, target_s=12.0):
     out = oracle.run(sample, steps, threads=threads, fields=("status",))
     wall = time.perf_counter() - t0
     done = int(out["steps_done"].sum())
+    # the one-system CPU case (an M dwarf with two Earth-mass planets, full
+    # effects), 1e5 steps on one core
+    from posidonius_amd import generators as gen
+    one = gen.config0()
+    t0 = time.perf_counter()
+    r0 = oracle.run(one, 100_000, threads=1, fields=("status",))
+    wall0 = time.perf_counter() - t0
     return {"value": done / wall, "unit": "system-steps/s", "cores": threads, "kind": "port",
             "sample": f"{n} systems of the same ensemble (first {n}), {steps} steps each, "
                       f"{threads} threads, one system per thread at a time; {wall:.1f} s wall",
-            "build": "g++ -O3 -ffp-contract=off (oracle/Makefile)"}
+            "build": "g++ -O3 -ffp-contract=off (oracle/Makefile)",
+            "host": {"cpu_model": cpu_model(), "logical_cpus": os.cpu_count(),
+                     "usable_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None},
+            "single_system": {"workload": "M dwarf + 2 Earth-mass planets, tides + flattening + Kidder GR, "
+                                          "dt 0.08 d", "steps": int(r0["steps_done"][0]), "wall_s": wall0,
+                              "cores": 1, "system_steps_per_s": float(r0["steps_done"][0]) / wall0}}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
