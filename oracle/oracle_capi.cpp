@@ -54,6 +54,25 @@ extern "C" {
 
 const char *oracle_last_error(void) { return g_err.c_str(); }
 
+// One universal-variable Kepler drift (kepler_individual_step, whfast.rs:676-876)
+// of a single body relative to a central mass with G M = mass_g, over dt; the
+// status bits it raises (timestep warning, bisection fallback, iteration cap)
+// are returned in *status.  Used by the tests to check the drift against the
+// closed-form two-body solution.
+int oracle_kepler_drift(double mass_g, double dt, const double *x, const double *v, double *out_x, double *out_v,
+                        uint32_t *status) {
+    auto *S = new oracle::System<double>();
+    S->alt[1].x = oracle::V3<double>{x[0], x[1], x[2]};
+    S->alt[1].v = oracle::V3<double>{v[0], v[1], v[2]};
+    oracle::kepler_individual_step(*S, 1, mass_g, dt);
+    const oracle::V3<double> X = S->alt[1].x, W = S->alt[1].v;
+    out_x[0] = X.x; out_x[1] = X.y; out_x[2] = X.z;
+    out_v[0] = W.x; out_v[1] = W.y; out_v[2] = W.z;
+    *status = S->status;
+    delete S;
+    return 0;
+}
+
 // Integrate every system for up to n_steps calls of WHFast::iterate (fewer if
 // it completes or freezes) on n_threads host threads, one system at a time
 // per thread.  If out_midpoint_iterations is non-NULL it receives, per system,
