@@ -1,0 +1,94 @@
+"""The oracle's protoplanetary-disk forces (src/effects/disk.rs:155-268,
+Alibert et al. 2013) on the part no reference fixture pins: the surface
+density (disk.rs:196-205), which the fixture's planets saturate.  Planets on
+circular orbits in the disk plane, the disk the only effect, a surface
+density low enough that min(1, 2 Sigma a^2 / m) is not saturated; the checks
+follow from the disk model itself, not from its code:
+
+* Sigma decays as exp(-t / lifetime), so the force does;
+* inside the inner edge there is no gas and no force;
+* with viscosity alpha c_s^2 / Omega, T proportional to r^-1/2, gas drift
+  -3 nu / (2 a) and a migration force -m v / tau with tau = -a / a_dot, the
+  force on a circular orbit is proportional to r^1/2 Sigma(r), with the
+  profile Sigma(r) ~ exp(-r / r_out) (1 - sqrt(r_in / r)) / r;
+* the force is anti-parallel to the velocity (inward migration; the
+  eccentricity and inclination damping terms vanish on circular planar
+  orbits), and the host takes the reaction;
+* once 2 Sigma a^2 / m exceeds 1 the force no longer depends on Sigma."""
+import numpy as np
+
+from posidonius_amd import abi, generators as gen
+from tests.oracle_runner import load_golden
+
+GM = 2.959122082855911e-04
+
+
+def disk_systems(radii, time=0.0, density_scale=1e-4, planet_mass=3e-6):
+    n = len(radii)
+    pos = np.zeros((n, 2, 3))
+    vel = np.zeros((n, 2, 3))
+    for i, r in enumerate(radii):
+        phi = 0.7 * i
+        pos[i, 1] = r * np.array([np.cos(phi), np.sin(phi), 0.0])
+        vel[i, 1] = np.sqrt(GM * (1.0 + planet_mass) / r) * np.array([-np.sin(phi), np.cos(phi), 0.0])
+    mass = np.tile([1.0, planet_mass], (n, 1))
+    radius = np.tile([gen.R_SUN, 1e-4 * gen.R_SUN], (n, 1))
+    roles = np.tile([abi.ROLE_DISK_CENTRAL, abi.ROLE_DISK_ORBITING], (n, 1)).astype(np.uint32)
+    zeros = np.zeros((n, 2))
+    spec = gen.build_spec(np.full(n, 2), mass, radius, np.full((n, 2), 0.3), pos, vel, np.zeros((n, 2, 3)), roles,
+                          zeros, zeros, zeros, zeros, 0.01, abi.COORD_DEMOCRATIC_HELIOCENTRIC, effects=abi.EFFECT_DISK)
+    fixture, _ = load_golden("test_disk-enabled_disk")
+    props = np.repeat(fixture.system["disk_properties"], n, axis=0).copy()
+    props[:, 4] *= density_scale
+    spec.system["disk_properties"] = props
+    spec.system["current_time"][:] = time
+    return spec, props[0]
+
+
+def planet_and_host_accelerations(oracle, spec):
+    acc, _ = oracle.additional_effects(spec)
+    return acc[1::2], acc[0::2]
+
+
+def test_disk_force_decays_with_the_disk_lifetime(oracle):
+    radii = np.array([0.05, 0.2, 0.7, 1.5])
+    s0, props = disk_systems(radii, 0.0)
+    s1, _ = disk_systems(radii, props[2])   # one lifetime later
+    a0, _ = planet_and_host_accelerations(oracle, s0)
+    a1, _ = planet_and_host_accelerations(oracle, s1)
+    assert np.all(np.linalg.norm(a0, axis=1) > 0)
+    np.testing.assert_allclose(a1, a0 * np.exp(-1.0), rtol=1e-12, atol=0)
+
+
+def test_disk_force_follows_the_surface_density_profile(oracle):
+    radii = np.geomspace(0.012, 3.0, 24)
+    spec, props = disk_systems(radii)
+    r_in, r_out = props[0], props[1]
+    a, host = planet_and_host_accelerations(oracle, spec)
+    v = spec.body["velocity"][1::2] - spec.body["velocity"][0::2]
+    mag = np.linalg.norm(a, axis=1)
+    sigma_shape = np.exp(-radii / r_out) * (1.0 - np.sqrt(r_in / radii)) / radii
+    ratio = mag / (np.sqrt(radii) * sigma_shape)
+    np.testing.assert_allclose(ratio, ratio[0], rtol=1e-9)
+    # anti-parallel to the (heliocentric) velocity, host reaction -m a / M
+    cosang = np.einsum("ij,ij->i", a, v) / (mag * np.linalg.norm(v, axis=1))
+    np.testing.assert_allclose(cosang, -1.0, atol=1e-12)
+    np.testing.assert_allclose(host, -3e-6 * a, rtol=1e-12, atol=0)
+
+
+def test_no_disk_force_inside_the_inner_edge(oracle):
+    spec, props = disk_systems(np.array([0.005, 0.009, 0.0099]))
+    a, host = planet_and_host_accelerations(oracle, spec)
+    assert np.all(a == 0) and np.all(host == 0)
+
+
+def test_saturated_migration_does_not_depend_on_the_surface_density(oracle):
+    radii = np.array([0.1, 0.3, 0.9])
+    s1, _ = disk_systems(radii, density_scale=1.0)     # 2 Sigma a^2 / m well above 1
+    s2, _ = disk_systems(radii, density_scale=2.0)
+    thin, _ = disk_systems(radii, density_scale=1e-4)
+    a1, _ = planet_and_host_accelerations(oracle, s1)
+    a2, _ = planet_and_host_accelerations(oracle, s2)
+    at, _ = planet_and_host_accelerations(oracle, thin)
+    np.testing.assert_array_equal(a1, a2)
+    assert np.all(np.linalg.norm(at, axis=1) < 1e-2 * np.linalg.norm(a1, axis=1))
