@@ -20,7 +20,8 @@ import numpy as np
 from posidonius_amd import abi, generators as gen
 from tests.oracle_runner import load_golden
 
-GM = 2.959122082855911e-04
+# G M_sun in AU^3 / day^2 with the reference's constants (src/constants.rs:35-61)
+GM = 6.67428e-11 / 1.49597870700e11 ** 3 * 1.9818e30 * 86400.0 ** 2
 
 
 def disk_systems(radii, time=0.0, density_scale=1e-4, planet_mass=3e-6):
