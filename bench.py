@@ -161,6 +161,27 @@ def cpu_baseline(spec, threads, target_s=12.0):
                               "cores": 1, "system_steps_per_s": float(r0["steps_done"][0]) / wall0}}
 
 
+def operator_rates(ens, n_sys, calls=5):
+    """The operator-level entry points the reference benchmarks per universe
+    (benches/simulation.rs:127 gravity_calculate_acceleration, 138
+    calculate_additional_effects): pos_gravity_acceleration and
+    pos_additional_effects over every system of the ensemble, each call
+    including the copy of its result to host memory."""
+    ens.gravity_acceleration()
+    ens.additional_effects()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        ens.gravity_acceleration()
+    tg = (time.perf_counter() - t0) / calls
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        ens.additional_effects()
+    te = (time.perf_counter() - t0) / calls
+    return {"gravity_systems_per_s": n_sys / tg, "additional_effects_systems_per_s": n_sys / te,
+            "gravity_ms_per_call": tg * 1e3, "additional_effects_ms_per_call": te * 1e3,
+            "note": "one call evaluates every system and copies the result to host memory"}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -184,6 +205,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-operators", action="store_true")
     ap.add_argument("--adjoint-systems", type=int, default=1024)
     ap.add_argument("--adjoint-steps", type=int, default=100000)
     ap.add_argument("--adjoint-segment", type=int, default=1000)
@@ -245,6 +267,11 @@ def main():
     readback_s = time.perf_counter() - t0
     frozen = int(np.count_nonzero(full_state["status"] & abi.STATUS_FATAL))
     del full_state
+    operators = None
+    if not args.no_operators:
+        operators = operator_rates(ens, n_sys)
+        log(f"operators: gravity {operators['gravity_systems_per_s']:.4g} systems/s, "
+            f"additional effects {operators['additional_effects_systems_per_s']:.4g} systems/s")
     ens.close()
     launches = math.ceil(args.steps / K)
     value = n_sys * args.steps / wall
@@ -332,6 +359,7 @@ def main():
         "plain_gpu": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
         "adjoint": adj,
+        "operators": operators,
         "frozen_systems": frozen,
         "pcie_inclusive": {"value": n_sys * args.steps / (wall + readback_s), "unit": "system-steps/s",
                            "readback_ms": readback_s * 1e3,
