@@ -216,7 +216,9 @@ typedef struct pos_options {
     int32_t device;            /* HIP device ordinal                          */
     int32_t mapping;           /* POS_MAPPING_*                               */
     int32_t group_size;        /* lanes per system for PACKED (0 = smallest power of two >= N,
-                                  -1 = exactly N, > 0 = that many, at least N) */
+                                  -1 = exactly N, > 0 = that many, at least N,
+                                  -2 = exactly N with systems of different sizes
+                                  sharing a wavefront, best-fit decreasing) */
     int32_t steps_per_launch;  /* K for PACKED (0 = default 64)               */
     int32_t use_graph;         /* capture chunk sequences in a hipGraph       */
     int32_t reserved[3];

@@ -362,13 +362,12 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
 __device__ inline void adjoint_load(const AdjArgs &Ar, AdjLds &L, AdjLane &A, int wave) {
     const DevEnsemble &E = Ar.E;
     const int lane = threadIdx.x;
-    const int G = E.wave_group[wave];
-    const int g = lane / G;
-    const int k = lane - g * G;
-    const bool gvalid = g < E.wave_count[wave];
-    const int s = E.wave_first[wave] + (gvalid ? g : 0);
+    const LaneGroup lg = lane_group(E, wave, lane);
+    const int k = lg.k;
+    const bool gvalid = lg.gvalid;
+    const int s = lg.sys;
     A.lane = lane;
-    A.gbase = g * G;
+    A.gbase = lg.gbase;
     A.k = k;
     A.sys = s;
     A.n = gvalid ? (E.body_offset[s + 1] - E.body_offset[s]) : 0;

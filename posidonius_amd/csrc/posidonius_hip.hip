@@ -34,14 +34,12 @@ namespace {
 template <int V = 0>
 __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, int wave) {
     const int lane = threadIdx.x;
-    const int G = E.wave_group[wave];
-    const int g = lane / G;
-    const int k = lane - g * G;
-    const int count = E.wave_count[wave];
-    const bool gvalid = g < count;
-    const int s = E.wave_first[wave] + (gvalid ? g : 0);
+    const LaneGroup lg = lane_group(E, wave, lane);
+    const int k = lg.k;
+    const bool gvalid = lg.gvalid;
+    const int s = lg.sys;
     B.lane = lane;
-    B.gbase = g * G;
+    B.gbase = lg.gbase;
     B.k = k;
     B.sys = s;
     B.n = gvalid ? (E.body_offset[s + 1] - E.body_offset[s]) : 0;
@@ -308,10 +306,11 @@ struct pos_ens {
     std::vector<int> body_offset_in;    // input CSR
     std::vector<int> n_per_sys_dev;
     // launch tables per mapping
-    std::vector<int> wave_first, wave_count, wave_group;
+    std::vector<int> wave_first, wave_count, wave_group, wave_sys;
     int variant_waves[N_VARIANTS][2] = {};   // [kernel variant] -> wave range [begin, end)
     std::vector<int> variant_dev;           // kernel variant of each device system
     int *d_wave_first = nullptr, *d_wave_count = nullptr, *d_wave_group = nullptr;   // capacity n_systems
+    int *d_wave_sys = nullptr;                                                        // capacity n_systems
     double *d_op_a = nullptr, *d_op_l = nullptr;   // operator outputs, [3][n_bodies]
     uint32_t *d_op_st = nullptr;                    // operator status, [n_systems]
     int steps_per_launch = 1;
@@ -389,8 +388,48 @@ int build_launch(pos_ens *h, const pos_options &o) {
     h->wave_first.clear();
     h->wave_count.clear();
     h->wave_group.clear();
+    h->wave_sys.clear();
     const int ns = h->n_systems;
-    if (o.mapping == POS_MAPPING_PLAIN) {
+    if (o.mapping == POS_MAPPING_PACKED && o.group_size == -2) {
+        // exactly N lanes per system, systems of one kernel variant packed
+        // into 64-lane wavefronts best-fit decreasing: largest systems first,
+        // each into the open wavefront with the least room that holds it
+        // (the device order is by variant, then by size).  A wavefront costs
+        // about what its largest system costs, so the small systems fill the
+        // lanes the large ones leave idle.
+        int s = 0;
+        while (s < ns) {
+            int e = s;
+            while (e < ns && h->variant_dev[e] == h->variant_dev[s]) ++e;
+            std::vector<std::vector<int>> bins;
+            std::vector<int> by_room[65];   // open wavefronts by free lanes
+            for (int d = e - 1; d >= s; --d) {
+                const int n = std::min(h->n_per_sys_dev[d], 64);
+                int r = n;
+                while (r <= 64 && by_room[r].empty()) ++r;
+                int b;
+                if (r > 64) {
+                    b = (int)bins.size();
+                    bins.emplace_back();
+                    r = 64;
+                } else {
+                    b = by_room[r].back();
+                    by_room[r].pop_back();
+                }
+                bins[b].push_back(d);
+                by_room[r - n].push_back(b);
+            }
+            // lightest wavefronts first here: the table is reversed below
+            for (size_t b = bins.size(); b-- > 0;) {
+                h->wave_first.push_back((int)h->wave_sys.size());
+                h->wave_count.push_back((int)bins[b].size());
+                h->wave_group.push_back(0);
+                h->wave_sys.insert(h->wave_sys.end(), bins[b].begin(), bins[b].end());
+            }
+            s = e;
+        }
+        h->steps_per_launch = o.steps_per_launch > 0 ? o.steps_per_launch : 64;
+    } else if (o.mapping == POS_MAPPING_PLAIN) {
         for (int s = 0; s < ns; ++s) {
             h->wave_first.push_back(s);
             h->wave_count.push_back(1);
@@ -433,19 +472,22 @@ int build_launch(pos_ens *h, const pos_options &o) {
         h->variant_waves[v][1] = 0;
     }
     for (int w = 0; w < nw; ++w) {
-        const int v = h->variant_dev[h->wave_first[w]];
+        const int first_sys = h->wave_group[w] > 0 ? h->wave_first[w] : h->wave_sys[h->wave_first[w]];
+        const int v = h->variant_dev[first_sys];
         h->variant_waves[v][0] = std::min(h->variant_waves[v][0], w);
         h->variant_waves[v][1] = std::max(h->variant_waves[v][1], w + 1);
     }
     int rc;
     if (!h->d_wave_first) {
         if ((rc = dalloc(h, &h->d_wave_first, ns)) || (rc = dalloc(h, &h->d_wave_count, ns)) ||
-            (rc = dalloc(h, &h->d_wave_group, ns)))
+            (rc = dalloc(h, &h->d_wave_group, ns)) || (rc = dalloc(h, &h->d_wave_sys, ns)))
             return rc;
     }
     if ((rc = upload(h, h->d_wave_first, h->wave_first)) || (rc = upload(h, h->d_wave_count, h->wave_count)) ||
-        (rc = upload(h, h->d_wave_group, h->wave_group)))
+        (rc = upload(h, h->d_wave_group, h->wave_group)) ||
+        (!h->wave_sys.empty() && (rc = upload(h, h->d_wave_sys, h->wave_sys))))
         return rc;
+    h->E.wave_sys = h->d_wave_sys;
     h->E.wave_first = h->d_wave_first;
     h->E.wave_count = h->d_wave_count;
     h->E.wave_group = h->d_wave_group;
@@ -1211,7 +1253,9 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
     };
     int max_n = 0;
     for (int n : h->n_per_sys_dev) max_n = std::max(max_n, n);
-    std::vector<int> groups = {0, -1};   // next power of two / exactly N lanes, per size class
+    // next power of two / exactly N lanes per size class / exactly N lanes
+    // with sizes mixed in a wavefront
+    std::vector<int> groups = {0, -1, -2};
     for (int g = max_n; g <= 16; ++g)
         if (64 % g != 0) groups.push_back(g);   // exact packing for sizes that do not divide 64
     std::vector<int> ks = {16, 64};

@@ -120,7 +120,10 @@ struct DevEnsemble {
     double *pair_ph;   // an orbiting body's own pair-dependent factor map entry, NaN = none
     const double *disk_props;
     // launch table: wave w covers device systems wave_first[w] .. +wave_count[w]-1
-    const int *wave_first, *wave_count, *wave_group;
+    // in groups of wave_group[w] lanes; a wave with wave_group[w] == 0 holds
+    // systems of different sizes, wave_sys[wave_first[w] ..
+    // +wave_count[w]-1], N lanes each, in that order
+    const int *wave_first, *wave_count, *wave_group, *wave_sys;
     int n_waves;
     // historic snapshots (optional): record r of system s is written at
     // hist[(r * HIST_FIELDS + f) * n_bodies + body], hist_count[s] records so far
@@ -160,6 +163,47 @@ struct WaveLds {
 };
 
 __device__ __forceinline__ void wsync() { __syncthreads(); }
+
+// The lane group of `lane` in wave `wave`: its device system, first lane and
+// this lane's index in it.  A lane past the wave's groups gets gvalid =
+// false, the wave's first system and its own lane as group start (so that
+// every LDS index it forms stays inside the wave's columns).
+struct LaneGroup {
+    int sys, gbase, k;
+    bool gvalid;
+};
+__device__ inline LaneGroup lane_group(const DevEnsemble &E, int wave, int lane) {
+    const int G = E.wave_group[wave];
+    const int count = E.wave_count[wave];
+    LaneGroup r;
+    if (G > 0) {
+        const int g = lane / G;
+        r.gvalid = g < count;
+        r.sys = E.wave_first[wave] + (r.gvalid ? g : 0);
+        r.gbase = g * G;
+        r.k = lane - g * G;
+        return r;
+    }
+    const int *list = E.wave_sys + E.wave_first[wave];
+    r.sys = list[0];
+    r.gbase = lane;
+    r.k = 0;
+    r.gvalid = false;
+    int base = 0;
+    for (int i = 0; i < count; ++i) {
+        const int s = list[i];
+        const int n = E.body_offset[s + 1] - E.body_offset[s];
+        if (lane < base + n) {
+            r.sys = s;
+            r.gbase = base;
+            r.k = lane - base;
+            r.gvalid = true;
+            break;
+        }
+        base += n;
+    }
+    return r;
+}
 
 // G m of the body in LDS slot l (the reference's mass_g = mass * K2)
 __device__ __forceinline__ double mg_of(const WaveLds &lds, int l) { return lds.m[l] * K2; }

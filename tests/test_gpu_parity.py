@@ -89,12 +89,15 @@ def test_configs_bitwise_against_oracle(Ensemble, oracle, cfg, n, steps):
     assert_same_state(out, ref, where=f"config {cfg}")
 
 
-@pytest.mark.parametrize("group_size,K,graph", [(0, 1, False), (0, 7, False), (8, 64, True), (16, 32, False), (3, 16, True), (-1, 64, True)])
+@pytest.mark.parametrize("group_size,K,graph", [(0, 1, False), (0, 7, False), (8, 64, True), (16, 32, False), (3, 16, True), (-1, 64, True),
+                                               (-2, 64, True), (-2, 16, False)])
 def test_launch_mappings_agree(Ensemble, oracle, group_size, K, graph):
     """Group size, steps per launch and graph replay do not change a bit."""
     spec = gen.config2(96) if group_size in (16, -1) else gen.config1(96)
     if group_size == 3:
         spec = gen.config1(96)
+    if group_size == -2:   # sizes 2-16 mixed in wavefronts
+        spec = gen.config2(256, seed=43, n_min=2, n_max=abi.MAX_BODIES)
     steps = 100
     ref = oracle.run(spec, steps, threads=8)
     log(f"mapping G={group_size} K={K} graph={graph}")
@@ -194,7 +197,8 @@ def gr_variant_ensemble():
     return spec.subset(np.random.default_rng(5).permutation(spec.n_systems))
 
 
-@pytest.mark.parametrize("mapping,group_size,graph", [("packed", -1, True), ("packed", 0, False), ("plain", 0, False)])
+@pytest.mark.parametrize("mapping,group_size,graph", [("packed", -1, True), ("packed", 0, False), ("plain", 0, False),
+                                                      ("packed", -2, False)])
 def test_gr_variants_against_oracle(Ensemble, oracle, mapping, group_size, graph):
     """Anderson 1975 and Newhall 1983 (general_relativity.rs:461-895) run in
     launches of their own; all four implementations in one ensemble agree
@@ -269,7 +273,8 @@ def extended_ensemble():
     return spec.subset(np.random.default_rng(9).permutation(spec.n_systems))
 
 
-@pytest.mark.parametrize("mapping,group_size,graph", [("packed", -1, True), ("packed", 0, False), ("plain", 0, False)])
+@pytest.mark.parametrize("mapping,group_size,graph", [("packed", -1, True), ("packed", 0, False), ("plain", 0, False),
+                                                      ("packed", -2, True)])
 def test_extended_effects_against_oracle(Ensemble, oracle, mapping, group_size, graph):
     """Disk forces (disk.rs:155-268), evolving-dissipation lag angles and pair
     factors (evolution.rs:548-567, constant_time_lag.rs:20-94) in their own
@@ -397,7 +402,7 @@ def test_ejection_and_close_encounter_freeze_their_systems(Ensemble, oracle):
     assert_same_state(out, ref, where="ejection / close encounter")
 
 
-@pytest.mark.parametrize("group_size", [-1, 0])
+@pytest.mark.parametrize("group_size", [-1, 0, -2])
 def test_largest_systems_against_oracle(Ensemble, oracle, group_size):
     """Systems of the maximum size (POS_MAX_BODIES = 16 bodies: four per
     wavefront) under both lane-group mappings."""
@@ -427,7 +432,7 @@ def _star_moved(spec, rng):
     return out, perm
 
 
-@pytest.mark.parametrize("mapping,group_size", [("packed", -1), ("packed", 0), ("plain", 0)])
+@pytest.mark.parametrize("mapping,group_size", [("packed", -1), ("packed", 0), ("plain", 0), ("packed", -2)])
 def test_fixed_effect_set_with_the_host_anywhere(Ensemble, oracle, mapping, group_size):
     """The fixed-effect-set kernel (tides + flattening + Kidder GR in
     heliocentric coordinates, pair terms hoisted out of the implicit-midpoint
