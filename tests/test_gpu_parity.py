@@ -449,3 +449,25 @@ def test_fixed_effect_set_with_the_host_anywhere(Ensemble, oracle, mapping, grou
         ens.step(steps)
         out = ens.get_state()
     assert_same_state(out, ref, where=f"host anywhere ({mapping}, G={group_size})")
+
+
+@pytest.mark.parametrize("n_max", [12, 16])
+def test_tune_restores_the_state_and_keeps_the_bits(Ensemble, oracle, n_max):
+    """pos_tune's trial launches run on the ensemble's own state and put it
+    back; the mapping it chooses continues to the oracle's bits. With sizes up
+    to 16 the size classes leave over a tenth of the lanes idle, so the mixed
+    packing is among the trials."""
+    spec = gen.config2(192, seed=7, n_min=2, n_max=n_max)
+    steps = 60
+    ref = oracle.run(spec, steps, threads=8)
+    log(f"tune: {spec.n_systems} systems of 2-{n_max} bodies")
+    with Ensemble(spec, mapping="packed", use_graph=True) as ens:
+        ens.step(20)
+        before = ens.get_state()
+        o = ens.tune(probe_steps=16)
+        log(f"tune chose G={o.group_size} K={o.steps_per_launch} graph={o.use_graph}")
+        assert o.mapping == abi.MAPPING_PACKED and o.steps_per_launch in (16, 64)
+        assert_same_state(ens.get_state(), before, where="after tune")
+        ens.step(steps - 20)
+        out = ens.get_state()
+    assert_same_state(out, ref, where=f"tuned G={o.group_size} K={o.steps_per_launch}")
