@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -266,6 +267,18 @@ __global__ __launch_bounds__(64) void effects_kernel(DevEnsemble E, double *out_
     }
 }
 
+// Per-body field from device order to the caller's order: component q of
+// device body d (stored q * n + d) goes to caller body perm[d], comps values
+// per body.  perm is a permutation of 0..n-1, so no two threads write the
+// same element.
+__global__ __launch_bounds__(256) void unpermute_kernel(const double *__restrict__ src, const int *__restrict__ perm, int n,
+                                                        int comps, double *__restrict__ dst) {
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= n) return;
+    const size_t c = (size_t)perm[d] * comps;
+    for (int q = 0; q < comps; ++q) dst[c + q] = src[(size_t)q * n + d];
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -313,6 +326,8 @@ struct pos_ens {
     int *d_wave_sys = nullptr;                                                        // capacity n_systems
     double *d_op_a = nullptr, *d_op_l = nullptr;   // operator outputs, [3][n_bodies]
     uint32_t *d_op_st = nullptr;                    // operator status, [n_systems]
+    double *d_unperm = nullptr;                     // per-body field in caller order, [n_bodies][3]
+    int *d_body_perm = nullptr;                     // body_perm on the device
     int steps_per_launch = 1;
     int grid = 0;
     // device allocations
@@ -367,6 +382,26 @@ template <class T> int download(pos_ens *h, std::vector<T> &dst, const T *src, s
     return POS_OK;
 }
 
+
+// Copy a per-body field (comps components, device order) to the caller's
+// host array in the caller's body order: reordered on the device, then one
+// copy.  Synchronous.
+int download_bodies(pos_ens *h, double *dst, const double *src, int comps) {
+    const int nb = h->n_bodies;
+    int rc;
+    if (!h->d_unperm) {
+        if ((rc = dalloc(h, &h->d_unperm, 3 * (size_t)nb)) || (rc = dalloc(h, &h->d_body_perm, nb)) ||
+            (rc = upload(h, h->d_body_perm, h->body_perm)))
+            return rc;
+    }
+    if (comps < 1 || comps > 3) return fail(POS_E_INVALID, "internal error: field width");
+    hipLaunchKernelGGL(unpermute_kernel, dim3((nb + 255) / 256), dim3(256), 0, h->stream, src, h->d_body_perm, nb, comps,
+                       h->d_unperm);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(dst, h->d_unperm, (size_t)comps * nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return POS_OK;
+}
 
 int next_pow2(int n) {
     int g = 1;
@@ -954,25 +989,8 @@ int32_t pos_get_state(pos_ens *h, pos_state *o) {
     const int ns = h->n_systems, nb = h->n_bodies;
     std::vector<double> buf;
     int rc;
-    auto vec3 = [&](double *dst, double *src) -> int {
-        if (!dst) return POS_OK;
-        int r = download(h, buf, src, 3 * (size_t)nb);
-        if (r) return r;
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        for (int db = 0; db < nb; ++db) {
-            const int b = h->body_perm[db];
-            for (int q = 0; q < 3; ++q) dst[3 * b + q] = buf[(size_t)q * nb + db];
-        }
-        return POS_OK;
-    };
-    auto vec1 = [&](double *dst, double *src) -> int {
-        if (!dst) return POS_OK;
-        int r = download(h, buf, src, nb);
-        if (r) return r;
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        for (int db = 0; db < nb; ++db) dst[h->body_perm[db]] = buf[db];
-        return POS_OK;
-    };
+    auto vec3 = [&](double *dst, double *src) -> int { return dst ? download_bodies(h, dst, src, 3) : POS_OK; };
+    auto vec1 = [&](double *dst, double *src) -> int { return dst ? download_bodies(h, dst, src, 1) : POS_OK; };
     if ((rc = vec3(o->position, E.x)) || (rc = vec3(o->velocity, E.v)) || (rc = vec3(o->acceleration, E.acc)) ||
         (rc = vec3(o->angular_momentum, E.L)) || (rc = vec3(o->spin, E.spin)) || (rc = vec3(o->velocity_error, E.err_v)) ||
         (rc = vec3(o->angular_momentum_error, E.err_L)) || (rc = vec1(o->radius, E.radius)) ||
@@ -1011,12 +1029,7 @@ int32_t pos_get_state(pos_ens *h, pos_state *o) {
         HIP_TRY(hipStreamSynchronize(h->stream));
         for (int ds = 0; ds < ns; ++ds) o->timestep_warning[h->sys_perm[ds]] = tw[ds];
     }
-    if (o->pair_dissipation_factor) {
-        std::vector<double> pf;
-        if ((rc = download(h, pf, E.pair_ph, nb))) return rc;
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        for (int db = 0; db < nb; ++db) o->pair_dissipation_factor[h->body_perm[db]] = pf[db];
-    }
+    if ((rc = vec1(o->pair_dissipation_factor, E.pair_ph))) return rc;
     return POS_OK;
 }
 
@@ -1163,12 +1176,9 @@ int32_t pos_gravity_acceleration(pos_ens *h, double *out_acc, uint32_t *out_stat
     uint32_t *d_st = h->d_op_st;
     hipLaunchKernelGGL(gravity_kernel, dim3(h->E.n_waves), dim3(64), 0, h->stream, h->E, d_out, d_st);
     HIP_TRY(hipGetLastError());
-    std::vector<double> a;
     std::vector<uint32_t> st;
-    if ((rc = download(h, a, d_out, 3 * (size_t)nb)) || (rc = download(h, st, d_st, ns))) return rc;
+    if ((rc = download_bodies(h, out_acc, d_out, 3)) || (rc = download(h, st, d_st, ns))) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
-    for (int db = 0; db < nb; ++db)
-        for (int q = 0; q < 3; ++q) out_acc[3 * h->body_perm[db] + q] = a[(size_t)q * nb + db];
     if (out_status)
         for (int ds = 0; ds < ns; ++ds) out_status[h->sys_perm[ds]] |= st[ds];
     return build_launch(h, saved);
@@ -1203,14 +1213,7 @@ int32_t pos_additional_effects(pos_ens *h, double *out_acc, double *out_dLdt) {
         }
         HIP_TRY(hipGetLastError());
     }
-    std::vector<double> a, l;
-    if ((rc = download(h, a, d_a, 3 * (size_t)nb)) || (rc = download(h, l, d_l, 3 * (size_t)nb))) return rc;
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    for (int db = 0; db < nb; ++db)
-        for (int q = 0; q < 3; ++q) {
-            out_acc[3 * h->body_perm[db] + q] = a[(size_t)q * nb + db];
-            out_dLdt[3 * h->body_perm[db] + q] = l[(size_t)q * nb + db];
-        }
+    if ((rc = download_bodies(h, out_acc, d_a, 3)) || (rc = download_bodies(h, out_dLdt, d_l, 3))) return rc;
     return build_launch(h, saved);
 }
 
@@ -1253,9 +1256,18 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
     };
     int max_n = 0;
     for (int n : h->n_per_sys_dev) max_n = std::max(max_n, n);
-    // next power of two / exactly N lanes per size class / exactly N lanes
-    // with sizes mixed in a wavefront
-    std::vector<int> groups = {0, -1, -2};
+    // next power of two / exactly N lanes per size class
+    std::vector<int> groups = {0, -1};
+    {
+        // exactly N lanes with sizes mixed in a wavefront, when the size
+        // classes leave more than a tenth of the lanes idle (it measures the
+        // same as the size classes when they fill the lanes)
+        std::map<int, int64_t> per_size;
+        int64_t lanes = 0, waves = 0;
+        for (int n : h->n_per_sys_dev) { ++per_size[n]; lanes += n; }
+        for (const auto &c : per_size) waves += (c.second + 64 / c.first - 1) / (64 / c.first);
+        if (10 * lanes < 9 * 64 * waves) groups.push_back(-2);
+    }
     for (int g = max_n; g <= 16; ++g)
         if (64 % g != 0) groups.push_back(g);   // exact packing for sizes that do not divide 64
     std::vector<int> ks = {16, 64};
