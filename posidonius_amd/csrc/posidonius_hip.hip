@@ -279,6 +279,16 @@ __global__ __launch_bounds__(256) void unpermute_kernel(const double *__restrict
     for (int q = 0; q < comps; ++q) dst[c + q] = src[(size_t)q * n + d];
 }
 
+// The inverse of unpermute_kernel: caller order (comps per body) to device
+// order (component q of device body d at q * n + d).
+__global__ __launch_bounds__(256) void permute_kernel(const double *__restrict__ src, const int *__restrict__ perm, int n,
+                                                      int comps, double *__restrict__ dst) {
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= n) return;
+    const size_t c = (size_t)perm[d] * comps;
+    for (int q = 0; q < comps; ++q) dst[(size_t)q * n + d] = src[c + q];
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -386,19 +396,41 @@ template <class T> int download(pos_ens *h, std::vector<T> &dst, const T *src, s
 // Copy a per-body field (comps components, device order) to the caller's
 // host array in the caller's body order: reordered on the device, then one
 // copy.  Synchronous.
-int download_bodies(pos_ens *h, double *dst, const double *src, int comps) {
-    const int nb = h->n_bodies;
+int body_staging(pos_ens *h) {
     int rc;
     if (!h->d_unperm) {
+        const int nb = h->n_bodies;
         if ((rc = dalloc(h, &h->d_unperm, 3 * (size_t)nb)) || (rc = dalloc(h, &h->d_body_perm, nb)) ||
             (rc = upload(h, h->d_body_perm, h->body_perm)))
             return rc;
     }
+    return POS_OK;
+}
+
+int download_bodies(pos_ens *h, double *dst, const double *src, int comps) {
+    const int nb = h->n_bodies;
+    int rc;
+    if ((rc = body_staging(h))) return rc;
     if (comps < 1 || comps > 3) return fail(POS_E_INVALID, "internal error: field width");
     hipLaunchKernelGGL(unpermute_kernel, dim3((nb + 255) / 256), dim3(256), 0, h->stream, src, h->d_body_perm, nb, comps,
                        h->d_unperm);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(dst, h->d_unperm, (size_t)comps * nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return POS_OK;
+}
+
+// The inverse of download_bodies: the caller's host array (caller body
+// order) copied once, then put into device order on the device.  Synchronous.
+int upload_bodies(pos_ens *h, double *dst, const double *src, int comps) {
+    const int nb = h->n_bodies;
+    int rc;
+    if ((rc = body_staging(h))) return rc;
+    if (comps < 1 || comps > 3) return fail(POS_E_INVALID, "internal error: field width");
+    HIP_TRY(hipMemcpyAsync(h->d_unperm, src, (size_t)comps * nb * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(permute_kernel, dim3((nb + 255) / 256), dim3(256), 0, h->stream, h->d_unperm, h->d_body_perm, nb,
+                       comps, dst);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(h->stream));
     return POS_OK;
 }
@@ -1040,27 +1072,8 @@ int32_t pos_set_state(pos_ens *h, const pos_state *in) {
     const int ns = h->n_systems, nb = h->n_bodies;
     std::vector<double> buf;
     int rc;
-    auto vec3 = [&](const double *src, double *dst) -> int {
-        if (!src) return POS_OK;
-        buf.assign(3 * (size_t)nb, 0.);
-        for (int db = 0; db < nb; ++db) {
-            const int b = h->body_perm[db];
-            for (int q = 0; q < 3; ++q) buf[(size_t)q * nb + db] = src[3 * b + q];
-        }
-        int r = upload(h, dst, buf);
-        if (r) return r;
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        return POS_OK;
-    };
-    auto vec1 = [&](const double *src, double *dst) -> int {
-        if (!src) return POS_OK;
-        buf.assign(nb, 0.);
-        for (int db = 0; db < nb; ++db) buf[db] = src[h->body_perm[db]];
-        int r = upload(h, dst, buf);
-        if (r) return r;
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        return POS_OK;
-    };
+    auto vec3 = [&](const double *src, double *dst) -> int { return src ? upload_bodies(h, dst, src, 3) : POS_OK; };
+    auto vec1 = [&](const double *src, double *dst) -> int { return src ? upload_bodies(h, dst, src, 1) : POS_OK; };
     if ((rc = vec3(in->position, E.x)) || (rc = vec3(in->velocity, E.v)) || (rc = vec3(in->acceleration, E.acc)) ||
         (rc = vec3(in->angular_momentum, E.L)) || (rc = vec3(in->spin, E.spin)) || (rc = vec3(in->velocity_error, E.err_v)) ||
         (rc = vec3(in->angular_momentum_error, E.err_L)) || (rc = vec1(in->radius, E.radius)) ||
@@ -1100,12 +1113,7 @@ int32_t pos_set_state(pos_ens *h, const pos_state *in) {
         if ((rc = upload(h, E.timestep_warning, tw))) return rc;
         HIP_TRY(hipStreamSynchronize(h->stream));
     }
-    if (in->pair_dissipation_factor) {
-        std::vector<double> pf(nb);
-        for (int db = 0; db < nb; ++db) pf[db] = in->pair_dissipation_factor[h->body_perm[db]];
-        if ((rc = upload(h, E.pair_ph, pf))) return rc;
-        HIP_TRY(hipStreamSynchronize(h->stream));
-    }
+    if ((rc = vec1(in->pair_dissipation_factor, E.pair_ph))) return rc;
     return POS_OK;
 }
 
