@@ -3,7 +3,8 @@
 // spin), one segment of recorded steps at a time.
 //
 // Mapping: the forward step's (one lane per body, one lane group per system,
-// several systems per wavefront, one wavefront per workgroup).  Each lane
+// several systems per wavefront, of one size or of mixed sizes, one
+// wavefront per workgroup).  Each lane
 // holds the cotangents of its body's position, velocity and spin angular
 // momentum and accumulates the gradients of its body's parameters.  The step
 // being reversed (src/integrator/whfast.rs:235-305) is

@@ -173,15 +173,15 @@ struct LaneGroup {
     bool gvalid;
 };
 __device__ inline LaneGroup lane_group(const DevEnsemble &E, int wave, int lane) {
-    const int G = E.wave_group[wave];
+    const int lanes = E.wave_group[wave];   // lanes per group, 0 = sizes mixed
     const int count = E.wave_count[wave];
     LaneGroup r;
-    if (G > 0) {
-        const int g = lane / G;
+    if (lanes > 0) {
+        const int g = lane / lanes;
         r.gvalid = g < count;
         r.sys = E.wave_first[wave] + (r.gvalid ? g : 0);
-        r.gbase = g * G;
-        r.k = lane - g * G;
+        r.gbase = g * lanes;
+        r.k = lane - g * lanes;
         return r;
     }
     const int *list = E.wave_sys + E.wave_first[wave];
