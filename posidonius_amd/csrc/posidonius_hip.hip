@@ -1172,11 +1172,8 @@ int32_t pos_gravity_acceleration(pos_ens *h, double *out_acc, uint32_t *out_stat
     if (!h || !out_acc) return fail(POS_E_INVALID, "NULL argument");
     HIP_TRY(hipSetDevice(h->device));
     const int ns = h->n_systems, nb = h->n_bodies;
-    pos_options saved = h->opts;
     int rc;
-    pos_options plain{};
-    plain.mapping = POS_MAPPING_PLAIN;
-    if ((rc = build_launch(h, plain))) return rc;
+    // one evaluation of every system over the current launch table
     if (!h->d_op_a && ((rc = dalloc(h, &h->d_op_a, 3 * (size_t)nb)) || (rc = dalloc(h, &h->d_op_l, 3 * (size_t)nb)) ||
                        (rc = dalloc(h, &h->d_op_st, ns))))
         return rc;
@@ -1189,18 +1186,15 @@ int32_t pos_gravity_acceleration(pos_ens *h, double *out_acc, uint32_t *out_stat
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (out_status)
         for (int ds = 0; ds < ns; ++ds) out_status[h->sys_perm[ds]] |= st[ds];
-    return build_launch(h, saved);
+    return POS_OK;
 }
 
 int32_t pos_additional_effects(pos_ens *h, double *out_acc, double *out_dLdt) {
     if (!h || !out_acc || !out_dLdt) return fail(POS_E_INVALID, "NULL argument");
     HIP_TRY(hipSetDevice(h->device));
     const int nb = h->n_bodies;
-    pos_options saved = h->opts;
     int rc;
-    pos_options plain{};
-    plain.mapping = POS_MAPPING_PLAIN;
-    if ((rc = build_launch(h, plain))) return rc;
+    // one evaluation of every system over the current launch table
     if (!h->d_op_a && ((rc = dalloc(h, &h->d_op_a, 3 * (size_t)nb)) || (rc = dalloc(h, &h->d_op_l, 3 * (size_t)nb)) ||
                        (rc = dalloc(h, &h->d_op_st, h->n_systems))))
         return rc;
@@ -1222,7 +1216,7 @@ int32_t pos_additional_effects(pos_ens *h, double *out_acc, double *out_dLdt) {
         HIP_TRY(hipGetLastError());
     }
     if ((rc = download_bodies(h, out_acc, d_a, 3)) || (rc = download_bodies(h, out_dLdt, d_l, 3))) return rc;
-    return build_launch(h, saved);
+    return POS_OK;
 }
 
 int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {

@@ -107,13 +107,19 @@ def test_launch_mappings_agree(Ensemble, oracle, group_size, K, graph):
     assert_same_state(out, ref, where=f"G={group_size} K={K} graph={graph}")
 
 
-def test_operators_against_oracle(Ensemble, oracle):
+@pytest.mark.parametrize("mapping,group_size,graph", [("packed", 0, False), ("plain", 0, False), ("packed", -2, True)])
+def test_operators_against_oracle(Ensemble, oracle, mapping, group_size, graph):
     """pos_gravity_acceleration / pos_additional_effects: the two operator-level
-    entry points the reference benchmarks (benches/simulation.rs:127, 138)."""
+    entry points the reference benchmarks (benches/simulation.rs:127, 138),
+    over each launch table; steps taken after them are unaffected."""
     spec = EnsembleSpec.concat([gen.config2(64), gen.config3(16), load_golden("test_integrator-whfast_jacobi")[0]])
-    with Ensemble(spec) as ens:
+    log(f"operators over {mapping} G={group_size} graph={graph}")
+    with Ensemble(spec, mapping=mapping, group_size=group_size, steps_per_launch=16, use_graph=graph) as ens:
         g_acc, g_st = ens.gravity_acceleration()
         e_acc, e_dl = ens.additional_effects()
+        ens.step(64)
+        after = ens.get_state()
+    assert_same_state(after, oracle.run(spec, 64, threads=8), where=f"steps after the operators ({mapping})")
     r_acc, r_st = oracle.gravity(spec)
     np.testing.assert_array_equal(g_acc, r_acc)
     np.testing.assert_array_equal(g_st, r_st)
