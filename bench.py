@@ -31,7 +31,11 @@ sys.path.insert(0, ROOT)
 # them.
 FLOPS_PER_SYSTEM_STEP = {2: 5978, 3: 11356, 4: 16747, 5: 22177, 6: 27614, 7: 33118, 8: 38616,
                          9: 44177, 10: 49766, 11: 55408, 12: 61076}
-FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix), AMD specification
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector, AMD specification (an FMA counts 2)
+# Without multiply-add contraction (the step is built with -ffp-contract=off,
+# as the reference never fuses) every operation is one add or multiply, and
+# the FP64 pipe retires half the flops it does on FMAs.
+FP64_NO_FMA_PEAK_TFLOPS = FP64_PEAK_TFLOPS / 2
 HBM_PEAK_GBS = 8000.0
 
 
@@ -73,21 +77,40 @@ def timed_steps(ens, steps, events):
     return wall, events.elapsed_ms()
 
 
-def measured_traffic(workload, steps_per_launch):
-    """HBM bytes per launch of the step kernel, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of the same workload and launch shape
+def measured_traffic(workload, launch_steps):
+    """HBM bytes of one step-kernel launch of `launch_steps` steps, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries of the same workload
     (profiles/*.md, `traffic_json:` lines, counters already scaled by the
-    calibration copy); None when no summary matches."""
+    calibration copy).  A launch moves a fixed amount (the state in and out,
+    the set-up around the step loop) plus an amount per step, so two
+    summaries of different launch lengths give both, and any launch length
+    is priced from them; one summary of exactly that length is used as is.
+    None when the summaries do not cover it."""
     import glob
+    recs = []
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_json:"):
                     rec = json.loads(line.split(":", 1)[1])
-                    if rec["workload"] == workload and rec["steps_per_launch"] == steps_per_launch:
+                    if rec["workload"] == workload:
                         rec["source"] = os.path.relpath(path, ROOT)
-                        return rec
-    return None
+                        recs.append(rec)
+    for rec in recs:
+        if rec["steps_per_launch"] == launch_steps:
+            return dict(rec, model="measured at this launch length")
+    lengths = sorted({r["steps_per_launch"] for r in recs})
+    if len(lengths) < 2:
+        return None
+    a = next(r for r in recs if r["steps_per_launch"] == lengths[0])
+    b = next(r for r in recs if r["steps_per_launch"] == lengths[-1])
+    per_step = (b["bytes_per_launch"] - a["bytes_per_launch"]) / (lengths[-1] - lengths[0])
+    fixed = a["bytes_per_launch"] - per_step * lengths[0]
+    return {"workload": workload, "steps_per_launch": launch_steps, "bytes_per_launch": fixed + per_step * launch_steps,
+            "fetch_scale": a.get("fetch_scale"), "write_scale": a.get("write_scale"),
+            "source": f"{a['source']}, {b['source']}",
+            "model": f"{fixed:.4g} B per launch + {per_step:.4g} B per step, from launches of "
+                     f"{lengths[0]} and {lengths[-1]} steps"}
 
 
 def adjoint_cost(n_systems, n_steps, segment_steps):
@@ -165,21 +188,27 @@ def operator_rates(ens, n_sys, calls=5):
     """The operator-level entry points the reference benchmarks per universe
     (benches/simulation.rs:127 gravity_calculate_acceleration, 138
     calculate_additional_effects): pos_gravity_acceleration and
-    pos_additional_effects over every system of the ensemble, each call
-    including the copy of its result to host memory."""
-    ens.gravity_acceleration()
-    ens.additional_effects()
-    t0 = time.perf_counter()
-    for _ in range(calls):
-        ens.gravity_acceleration()
-    tg = (time.perf_counter() - t0) / calls
-    t0 = time.perf_counter()
-    for _ in range(calls):
-        ens.additional_effects()
-    te = (time.perf_counter() - t0) / calls
-    return {"gravity_systems_per_s": n_sys / tg, "additional_effects_systems_per_s": n_sys / te,
-            "gravity_ms_per_call": tg * 1e3, "additional_effects_ms_per_call": te * 1e3,
-            "note": "one call evaluates every system and copies the result to host memory"}
+    pos_additional_effects over every system of the ensemble.  The kernels
+    are timed with HIP events on the library's stream (the handle's own
+    pos_get_launch_stats); the whole call, which also reorders the result
+    into the caller's body order and copies it to host memory, is timed on
+    the host clock and reported beside it."""
+    out = {}
+    for name, call in (("gravity", ens.gravity_acceleration), ("additional_effects", ens.additional_effects)):
+        call()
+        kernel = []
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            call()
+            kernel.append(ens.launch_stats()["last_operator_kernel_ms"])
+        wall = (time.perf_counter() - t0) / calls
+        k_ms = sum(kernel) / len(kernel)
+        out[name] = {"kernel_ms": k_ms, "kernel_systems_per_s": n_sys / (k_ms * 1e-3),
+                     "call_ms": wall * 1e3, "call_systems_per_s": n_sys / wall,
+                     "copy_and_reorder_ms": wall * 1e3 - k_ms}
+    out["note"] = ("one call evaluates every system; kernel_* is the device time of the evaluation kernels, "
+                   "call_* adds the reordering into the caller's body order and the copy to pageable host memory")
+    return out
 
 
 def cpu_model():
@@ -261,7 +290,12 @@ def main():
     ens.step(args.warmup)
     ens.synchronize()
     log(f"timing {args.steps} steps")
+    before = ens.launch_stats()
     wall, ev_ms = timed_steps(ens, args.steps, events)
+    after = ens.launch_stats()
+    direct = int(after["kernel_launches"] - before["kernel_launches"])
+    replays = int(after["graph_replays"] - before["graph_replays"])
+    launches = direct + replays * int(after["launches_per_replay"])
     t0 = time.perf_counter()
     full_state = ens.get_state()     # every field, device -> host
     readback_s = time.perf_counter() - t0
@@ -270,13 +304,15 @@ def main():
     operators = None
     if not args.no_operators:
         operators = operator_rates(ens, n_sys)
-        log(f"operators: gravity {operators['gravity_systems_per_s']:.4g} systems/s, "
-            f"additional effects {operators['additional_effects_systems_per_s']:.4g} systems/s")
+        log(f"operators (kernels): gravity {operators['gravity']['kernel_systems_per_s']:.4g} systems/s, "
+            f"additional effects {operators['additional_effects']['kernel_systems_per_s']:.4g} systems/s")
     ens.close()
-    launches = math.ceil(args.steps / K)
     value = n_sys * args.steps / wall
-    kernel_ms = ev_ms / launches
-    flops_per_launch = flops_per_step * min(K, args.steps)
+    # the dominant kernel: one launch per variant present (one here), each of
+    # launch_steps steps; the average launch is the timed region / launches
+    launch_steps = min(K, args.steps)
+    kernel_ms = ev_ms / max(1, launches)
+    flops_per_launch = flops_per_step * launch_steps
     achieved_tflops = flops_per_launch / (kernel_ms * 1e-3) / 1e12
     log(f"batched: {value:.4g} system-steps/s, {wall / args.steps * 1e3:.3f} ms/step, "
         f"kernel {kernel_ms:.3f} ms/launch, {achieved_tflops:.2f} TFLOP/s algorithmic")
@@ -313,7 +349,7 @@ def main():
 
     mean_n = float(counts.mean())
     workload_key = {1: "uniform", 2: "uneven", 3: "binary", 4: "adjoint"}[args.config] + str(n_sys)
-    traffic = measured_traffic(workload_key, K) if args.steps >= K else None
+    traffic = measured_traffic(workload_key, launch_steps) if args.steps % launch_steps == 0 else None
     result = {
         "metric": "ensemble system-steps/s FP64 full effects (batched vs plain GPU); adjoint/fwd cost",
         "value": value,
@@ -335,23 +371,32 @@ def main():
                         if n_sys == defaults[args.config] else f"config {args.config} with {n_sys} systems",
             "systems": n_sys, "mean_bodies": mean_n, "effects": "CTL tides + oblate flattening + Kidder GR + spin",
             "coordinates": "Jacobi" if args.config == 3 else "DemocraticHeliocentric",
-            "group_size": o.group_size, "steps_per_launch": K, "hip_graph": bool(o.use_graph),
+            "group_size": o.group_size, "steps_per_launch": K,
+            "timed_region": {"steps": args.steps, "kernel_launches_direct": direct, "graph_replays": replays,
+                             "launches_per_replay": int(after["launches_per_replay"]) if replays else 0,
+                             "steps_per_kernel_launch": launch_steps},
+            "hip_graph": replays > 0,
         },
         "roofline": {
-            "bound": "mfma",
-            "note": "FP64 compute bound (VALU; nothing in the step is a contraction, no MFMA is used); "
-                    "peak is MI355X FP64 78.6 TFLOP/s",
+            "bound": "fp64-valu",
+            "note": "FP64 vector-ALU bound: nothing in the step is a contraction, no MFMA is used. peak is the "
+                    "MI355X FP64 vector peak (an FMA counts 2); peak_no_fma is the ceiling of separately rounded "
+                    "adds and multiplies, which is what the step executes (-ffp-contract=off, as the reference)",
             "achieved": achieved_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+            "peak_no_fma": FP64_NO_FMA_PEAK_TFLOPS,
+            "frac_no_fma": achieved_tflops / FP64_NO_FMA_PEAK_TFLOPS,
             "traffic": traffic["bytes_per_launch"] if traffic else None,
             "traffic_source": (f"rocprofv3 FETCH_SIZE x{traffic.get('fetch_scale', 2.0)} + WRITE_SIZE x"
-                               f"{traffic.get('write_scale', 1.0)} per launch ({traffic['source']})") if traffic else None,
+                               f"{traffic.get('write_scale', 1.0)} per launch of {launch_steps} steps, "
+                               f"{traffic['model']} ({traffic['source']})") if traffic else None,
             "achieved_hbm_gbs": (traffic["bytes_per_launch"] / (kernel_ms * 1e-3) / 1e9) if traffic else None,
             "hbm_peak_gbs": HBM_PEAK_GBS,
             "kernel": "whfast_steps_kernel",
             "kernel_ms_per_launch": kernel_ms,
+            "steps_per_launch": launch_steps,
             "flops_per_launch": flops_per_launch,
             "flops_per_system_step_mean": flops_per_step / n_sys,
         },

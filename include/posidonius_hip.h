@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define POS_ABI_VERSION 3
+#define POS_ABI_VERSION 4
 
 /* Largest body count per system.  The reference caps a universe at
  * MAX_PARTICLES = 10 (src/constants.rs:3); the uneven-size ensemble needs 12.
@@ -179,6 +179,14 @@ typedef struct pos_ensemble_desc {
      * id * MAX_PARTICLES + host_id of universe.pair_dependent_scaled_
      * dissipation_factor (constant_time_lag.rs:96-165, 188-194) */
     const double   *pair_dissipation_factor;   /* [nb] */
+    /* Roche radii of a resumed system (optional): the radius of body b's pair
+     * with body j of its system at roche_radius[b * POS_MAX_BODIES + j]
+     * (universe.roche_radiuses, computed once by initialize_physical_values at
+     * t = 0 and carried by the snapshot, universe.rs:177-196, output.rs:187-193).
+     * Systems whose current_time is 0 compute them from the radii at t = 0 and
+     * ignore this array; a resumed system without it (or a NaN entry) gets
+     * them from its current radii. */
+    const double   *roche_radius;              /* [nb * POS_MAX_BODIES] */
 } pos_ensemble_desc;
 
 /*
@@ -205,6 +213,8 @@ typedef struct pos_state {
     int32_t  *timestep_warning;      /* [n_systems] WHFast.timestep_warning   */
     double   *pair_dissipation_factor; /* [nb] map entry of an evolving-dissipation
                                           orbiting body, NaN = none (see desc) */
+    double   *roche_radius;          /* [nb * POS_MAX_BODIES] the Roche radii in use,
+                                        laid out as in the desc               */
 } pos_state;
 
 /* Launch mapping chosen by the autotuner or by the caller. */
@@ -266,6 +276,23 @@ int32_t pos_get_stream(const pos_ens *ens, void **stream);
 
 /* The launch mapping currently in use (after pos_tune or the heuristic). */
 int32_t pos_get_options(const pos_ens *ens, pos_options *out);
+
+/* What the handle has executed since pos_create, for a caller that times a
+ * region and must say what ran inside it (read before and after).  The
+ * autotuner's probe launches are not counted.  No counterpart in the
+ * reference (which has no device). */
+typedef struct pos_launch_stats {
+    int64_t steps;                   /* ensemble steps asked of pos_step          */
+    int64_t kernel_launches;         /* step-kernel launches made directly         */
+    int64_t graph_replays;           /* hipGraph replays                           */
+    int64_t launches_per_replay;     /* step-kernel launches one replay holds      */
+    double  last_operator_kernel_ms; /* device time of the kernels of the last
+                                        pos_gravity_acceleration or
+                                        pos_additional_effects call (HIP events on
+                                        the handle's stream), without the copy of
+                                        its results                                */
+} pos_launch_stats;
+int32_t pos_get_launch_stats(const pos_ens *ens, pos_launch_stats *out);
 
 /* Autotuner: times candidate (group size, steps per launch) mappings on a
  * copy of the ensemble for `probe_steps` steps each, keeps the fastest, and

@@ -41,6 +41,10 @@ template <class R> void store_state(const oracle::System<R> &S, int b0, int s, p
         if (o->pair_dissipation_factor)
             o->pair_dissipation_factor[b] = P.ph_set ? val(P.ph) : std::numeric_limits<double>::quiet_NaN();
     }
+    if (o->roche_radius)
+        for (int i = 0; i < S.n; ++i)
+            for (int j = 0; j < oracle::MAXB; ++j)
+                o->roche_radius[(size_t)(b0 + i) * POS_MAX_BODIES + j] = (j < S.n && j != i) ? S.roche[i * S.n + j] : 0.;
     if (o->timestep_warning) o->timestep_warning[s] = S.timestep_warning;
     if (o->current_time) o->current_time[s] = val(S.t);
     if (o->last_historic_snapshot_time) o->last_historic_snapshot_time[s] = S.last_hist;
@@ -94,7 +98,7 @@ int oracle_run(const pos_ensemble_desc *d, int64_t n_steps, int32_t n_threads, p
                 if (failed.compare_exchange_strong(expected, s)) first_err = err;
                 continue;
             }
-            if (S->t == 0.) oracle::initialize_physical_values(*S);
+            oracle::start_system(*S, d, s);
             int64_t k = 0;
             while (k < n_steps) {
                 ++k;
@@ -129,7 +133,7 @@ int oracle_gravity(const pos_ensemble_desc *d, double *out_acc, uint32_t *out_st
     for (int s = 0; s < d->n_systems; ++s) {
         std::string err;
         if (!oracle::load_system(d, s, *S, &err)) { g_err = err; delete S; return -1; }
-        if (S->t == 0.) oracle::initialize_physical_values(*S);
+        oracle::start_system(*S, d, s);
         const int ignore = S->coord == POS_COORD_JACOBI ? oracle::IGNORE_WHFAST_ONE : oracle::IGNORE_WHFAST_TWO;
         oracle::gravity_calculate_acceleration(*S, ignore);
         const int b0 = d->body_offset[s];
@@ -151,7 +155,7 @@ int oracle_additional_effects(const pos_ensemble_desc *d, double *out_acc, doubl
     for (int s = 0; s < d->n_systems; ++s) {
         std::string err;
         if (!oracle::load_system(d, s, *S, &err)) { g_err = err; delete S; return -1; }
-        if (S->t == 0.) oracle::initialize_physical_values(*S);
+        oracle::start_system(*S, d, s);
         const bool integrate_spin = S->tides || S->rf || S->evolution || (S->gr && S->gr_impl == POS_GR_KIDDER1995);
         oracle::inertial_to_heliocentric(*S);
         oracle::calculate_additional_effects(*S, S->t, false, integrate_spin, true);
@@ -173,7 +177,7 @@ int oracle_flops(const pos_ensemble_desc *d, int64_t n_steps, double *out_flops,
     for (int s = 0; s < d->n_systems; ++s) {
         std::string err;
         if (!oracle::load_system(d, s, *S, &err)) { g_err = err; delete S; return -1; }
-        if (oracle::val(S->t) == 0.) oracle::initialize_physical_values(*S);
+        oracle::start_system(*S, d, s);
         oracle::FlopCount::count = 0;
         int64_t k = 0;
         while (k < n_steps) {
@@ -238,7 +242,7 @@ int oracle_tangent(const pos_ensemble_desc *d, int64_t n_steps, int32_t n_thread
                     if (d_k2f) P.k2f.d[k] = d_k2f[o1];
                 }
             }
-            if (oracle::val(S->t) == 0.) oracle::initialize_physical_values(*S);
+            oracle::start_system(*S, d, s);
             int64_t k = 0;
             while (k < n_steps) {
                 ++k;

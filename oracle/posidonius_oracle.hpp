@@ -31,9 +31,10 @@
 //   * loops the reference leaves unbounded (Stumpff argument reduction on an
 //     infinite argument, Kepler bisection on NaN brackets) are capped and
 //     raise POS_STATUS_ITERATION_CAP;
-//   * a system must have every effect host equal to its most massive body,
-//     and when tides / flattening are on every other body must be an
-//     orbiting body of that effect.  Other layouts are rejected at load;
+//   * a system must have its tidal, flattening and disk hosts equal to its
+//     most massive body (the reference also allows other hosts), and
+//     Anderson 1975 / Newhall 1983 need every other body to be a GR orbiting
+//     body.  Other layouts are rejected at load;
 //   * the pair-dependent factor map holds only what the step reads: the
 //     host's entries are recomputed before every use, the orbiting bodies'
 //     entries persist (loaded from and returned to the caller);
@@ -199,10 +200,18 @@ template <class R> struct Body {
     R k2;         // CTL love number
     R k2f;        // oblate-spheroid (fluid) love number
     R wind_k, wind_sat, wind_sat2;
-    // tides internal / output
+    // tides coordinates, internal, output.  The coordinates are copied from
+    // the heliocentric ones for the host and the tidal orbiting bodies only
+    // (tides/common.rs:202-217); every other body keeps the zeros it was
+    // created with, and so do its internal values.
+    V3<R> td_x, td_v;
+    R td_d, td_vr;
     R td_rs_star, td_rs_planet, td_orth_star, td_orth_planet, td_radial, td_diss_pm;
     V3<R> td_acc, td_dLdt;
-    // flattening internal / output
+    // flattening coordinates (rotational_flattening/common.rs:152-163, same
+    // rule as the tides), internal, output
+    V3<R> rf_x;
+    R rf_d;
     R rf_rs_star, rf_rs_planet, rf_radial, rf_fac_star, rf_fac_planet, rf_orth_star, rf_orth_planet;
     V3<R> rf_acc, rf_dLdt;
     // general relativity
@@ -331,22 +340,43 @@ inline bool load_system(const pos_ensemble_desc *d, int s, System<R> &S, std::st
             }
         }
     }
-    // Effect hosts must coincide with the most massive body (hosts.most_massive.all).
-    const uint32_t hr = S.p[S.host].roles;
-    if (S.tides && !(hr & POS_ROLE_TIDES_CENTRAL)) { if (err) *err = "tidal host is not the most massive body"; return false; }
-    if (S.rf && !(hr & POS_ROLE_FLAT_CENTRAL)) { if (err) *err = "flattening host is not the most massive body"; return false; }
-    if (S.gr && !(hr & POS_ROLE_GR_CENTRAL)) { if (err) *err = "GR host is not the most massive body"; return false; }
-    if (S.disk && !(hr & POS_ROLE_DISK_CENTRAL)) { if (err) *err = "disk host is not the most massive body"; return false; }
+    // disable_unnecessary_effects (universe.rs:699-764): an effect no body
+    // carries is switched off; find_indices (900-1006): one central body per
+    // effect, the GR host is the most massive body, tides and flattening
+    // share their host.
+    int host_of[4] = {-1, -1, -1, -1};
+    const uint32_t central[4] = {POS_ROLE_TIDES_CENTRAL, POS_ROLE_FLAT_CENTRAL, POS_ROLE_GR_CENTRAL, POS_ROLE_DISK_CENTRAL};
+    uint32_t any = 0;
+    bool evolving = false;
     for (int i = 0; i < n; ++i) {
-        if (i == S.host) continue;
-        const uint32_t r = S.p[i].roles;
-        if (r & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_FLAT_CENTRAL | POS_ROLE_GR_CENTRAL | POS_ROLE_DISK_CENTRAL)) {
-            if (err) *err = "a second central body";
-            return false;
-        }
-        if (S.tides && !(r & POS_ROLE_TIDES_ORBITING)) { if (err) *err = "tides on but a body is not a tidal orbiting body"; return false; }
-        if (S.rf && !(r & POS_ROLE_FLAT_ORBITING)) { if (err) *err = "flattening on but a body is not an orbiting body"; return false; }
+        any |= S.p[i].roles;
+        evolving = evolving || S.ev[i].type != POS_EVO_NON_EVOLVING;
+        for (int e = 0; e < 4; ++e)
+            if (S.p[i].roles & central[e]) {
+                if (host_of[e] >= 0) { if (err) *err = "only one central body is allowed per effect"; return false; }
+                host_of[e] = i;
+            }
     }
+    S.tides = S.tides && host_of[0] >= 0;
+    S.rf = S.rf && host_of[1] >= 0;
+    S.gr = S.gr && host_of[2] >= 0;
+    S.disk = S.disk && host_of[3] >= 0;
+    S.wind = S.wind && (any & POS_ROLE_WIND);
+    S.evolution = S.evolution && evolving;
+    if (S.tides && S.rf && host_of[0] != host_of[1]) { if (err) *err = "tidal and flattening hosts differ"; return false; }
+    if (S.gr && host_of[2] != S.host) { if (err) *err = "the GR host is not the most massive body"; return false; }
+    // not restated: a tidal, flattening or disk host other than the most
+    // massive body (hosts.most_massive.all false, universe.rs:386-419), and
+    // Anderson / Newhall with a body whose GR is Disabled (they read every
+    // body's GR coordinates, which the reference leaves stale for it)
+    if ((S.tides || S.rf) && host_of[S.tides ? 0 : 1] != S.host) { if (err) *err = "tidal host is not the most massive body"; return false; }
+    if (S.disk && host_of[3] != S.host) { if (err) *err = "disk host is not the most massive body"; return false; }
+    if (S.gr && S.gr_impl != POS_GR_KIDDER1995)
+        for (int i = 0; i < n; ++i)
+            if (i != S.host && !(S.p[i].roles & POS_ROLE_GR_ORBITING)) {
+                if (err) *err = "Anderson1975 / Newhall1983 with a body whose GR is Disabled";
+                return false;
+            }
     return true;
 }
 
@@ -422,6 +452,25 @@ template <class R> inline void initialize_physical_values(System<R> &S) {
     calculate_roche_radiuses(S);
 }
 
+// A fresh system (t = 0) runs initialize_physical_values (output.rs:187-193);
+// a resumed one keeps the Roche radii its snapshot carries
+// (universe.roche_radiuses); entries it lacks (NaN) come from its current radii.
+template <class R> inline void start_system(System<R> &S, const pos_ensemble_desc *d, int s) {
+    if (val(S.t) == 0.) {
+        initialize_physical_values(S);
+    } else {
+        calculate_roche_radiuses(S);   // the current radii, for entries not carried
+        if (d->roche_radius) {
+            const int b0 = d->body_offset[s];
+            for (int i = 0; i < S.n; ++i)
+                for (int j = 0; j < S.n; ++j) {
+                    const double r = d->roche_radius[(size_t)(b0 + i) * POS_MAX_BODIES + j];
+                    if (i != j && !std::isnan(r)) S.roche[i * S.n + j] = r;
+                }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // universe.rs:318-351
 // ---------------------------------------------------------------------------
@@ -452,26 +501,32 @@ enum IgnoreTerms { IGNORE_NONE = 0, IGNORE_WHFAST_ONE = 1, IGNORE_WHFAST_TWO = 2
 // tides/common.rs:143-170 (with copy_heliocentric_coordinates, 202-217)
 template <class R> inline void tides_initialize(System<R> &S) {
     Body<R> &H = S.p[S.host];
+    H.td_x = H.hx; H.td_v = H.hv; H.td_d = H.hdist; H.td_vr = H.hvr;
     H.td_rs_star = R(0.); H.td_rs_planet = R(0.);
     H.td_acc = v3zero<R>(); H.td_dLdt = v3zero<R>();
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
-        P.td_rs_star = P.hx.x * H.spin.x + P.hx.y * H.spin.y + P.hx.z * H.spin.z;
-        P.td_rs_planet = P.hx.x * P.spin.x + P.hx.y * P.spin.y + P.hx.z * P.spin.z;
+        if (!(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
+        P.td_x = P.hx; P.td_v = P.hv; P.td_d = P.hdist; P.td_vr = P.hvr;
+        P.td_rs_star = P.td_x.x * H.spin.x + P.td_x.y * H.spin.y + P.td_x.z * H.spin.z;
+        P.td_rs_planet = P.td_x.x * P.spin.x + P.td_x.y * P.spin.y + P.td_x.z * P.spin.z;
         P.td_acc = v3zero<R>(); P.td_dLdt = v3zero<R>();
     }
 }
 // rotational_flattening/common.rs:93-120
 template <class R> inline void flattening_initialize(System<R> &S) {
     Body<R> &H = S.p[S.host];
+    H.rf_x = H.hx; H.rf_d = H.hdist;
     H.rf_rs_star = R(0.); H.rf_rs_planet = R(0.);
     H.rf_acc = v3zero<R>(); H.rf_dLdt = v3zero<R>();
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
-        P.rf_rs_star = P.hx.x * H.spin.x + P.hx.y * H.spin.y + P.hx.z * H.spin.z;
-        P.rf_rs_planet = P.hx.x * P.spin.x + P.hx.y * P.spin.y + P.hx.z * P.spin.z;
+        if (!(P.roles & POS_ROLE_FLAT_ORBITING)) continue;
+        P.rf_x = P.hx; P.rf_d = P.hdist;
+        P.rf_rs_star = P.rf_x.x * H.spin.x + P.rf_x.y * H.spin.y + P.rf_x.z * H.spin.z;
+        P.rf_rs_planet = P.rf_x.x * P.spin.x + P.rf_x.y * P.spin.y + P.rf_x.z * P.spin.z;
         P.rf_acc = v3zero<R>(); P.rf_dLdt = v3zero<R>();
     }
 }
@@ -539,7 +594,7 @@ template <class R> inline void pair_dependent_dissipation_factors(System<R> &S) 
             if (i == S.host) continue;
             const Body<R> &P = S.p[i];
             if (!(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
-            hp_set[i] = excited_pair_factor(H.mg + P.mg, P.hx, P.hv, host_norm_spin, H.radius, H.lag, H.sigma_scale,
+            hp_set[i] = excited_pair_factor(H.mg + P.mg, P.td_x, P.td_v, host_norm_spin, H.radius, H.lag, H.sigma_scale,
                                             H.sigma_factor, hp[i]);
         }
     }
@@ -548,7 +603,7 @@ template <class R> inline void pair_dependent_dissipation_factors(System<R> &S) 
         Body<R> &P = S.p[i];
         if (!S.ev[i].dynamical() || !(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
         R v;
-        if (excited_pair_factor(H.mg + P.mg, P.hx, P.hv, sqrt_r(P.spin2), P.radius, P.lag, P.sigma_scale,
+        if (excited_pair_factor(H.mg + P.mg, P.td_x, P.td_v, sqrt_r(P.spin2), P.radius, P.lag, P.sigma_scale,
                                 P.sigma_factor, v)) {
             P.ph = v;
             P.ph_set = true;
@@ -570,13 +625,15 @@ template <class R> inline void tides_orthogonal(System<R> &S) {
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
-        R d7 = pw7(P.hdist);
+        if (!(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
+        R d7 = pw7(P.td_d);
         P.td_orth_star = R(4.5) * pw2(P.m) * pw10(H.radius) * P.sigma_host / d7;
     }
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
-        R d7 = pw7(P.hdist);
+        if (!(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
+        R d7 = pw7(P.td_d);
         P.td_orth_planet = R(4.5) * pw2(H.m) * pw10(P.radius) * P.sigma_pair / d7;
     }
 }
@@ -587,14 +644,16 @@ template <class R> inline void flattening_orthogonal(System<R> &S) {
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
+        if (!(P.roles & POS_ROLE_FLAT_ORBITING)) continue;
         P.rf_fac_star = P.m * host_k2f * H.spin2 * pw5(H.radius) / R(6.);
-        P.rf_orth_star = R(-6.) * P.rf_fac_star * P.rf_rs_star / (H.spin2 * pw5(P.hdist));
+        P.rf_orth_star = R(-6.) * P.rf_fac_star * P.rf_rs_star / (H.spin2 * pw5(P.rf_d));
     }
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
+        if (!(P.roles & POS_ROLE_FLAT_ORBITING)) continue;
         P.rf_fac_planet = H.m * P.k2f * P.spin2 * pw5(P.radius) / R(6.);
-        P.rf_orth_planet = R(-6.) * P.rf_fac_planet * P.rf_rs_planet / (P.spin2 * pw5(P.hdist));
+        P.rf_orth_planet = R(-6.) * P.rf_fac_planet * P.rf_rs_planet / (P.spin2 * pw5(P.rf_d));
     }
 }
 // constant_time_lag.rs:266-307
@@ -604,9 +663,10 @@ template <class R> inline void tides_radial(System<R> &S) {
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
+        if (!(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
         const R pm2 = P.m * P.m;
-        R conservative = R(-3.0 * K2) / pw7(P.hdist) * (pm2 * pw5(H.radius) * H.k2 + host_mass_2 * pw5(P.radius) * P.k2);
-        R factor1 = R(-13.5) * P.hvr / pw8(P.hdist);
+        R conservative = R(-3.0 * K2) / pw7(P.td_d) * (pm2 * pw5(H.radius) * H.k2 + host_mass_2 * pw5(P.radius) * P.k2);
+        R factor1 = R(-13.5) * P.td_vr / pw8(P.td_d);
         R term1 = pm2 * pw10(H.radius) * P.sigma_host;
         R term2 = host_mass_2 * pw10(P.radius) * P.sigma_pair;
         P.td_diss_pm = factor1 * term2;
@@ -623,18 +683,20 @@ template <class R> inline void tides_acceleration(System<R> &S) {
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
-        const R d = P.hdist;
-        const R factor3 = P.td_radial + (P.td_orth_star + P.td_orth_planet) * P.hvr / d;
+        if (!(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
+        const R d = P.td_d;
+        const V3<R> &x = P.td_x, &u = P.td_v;
+        const R factor3 = P.td_radial + (P.td_orth_star + P.td_orth_planet) * P.td_vr / d;
         V3<R> F;
-        F.x = factor3 * P.hx.x / d
-            + P.td_orth_star / d * (hs.y * P.hx.z - hs.z * P.hx.y - P.hv.x)
-            + P.td_orth_planet / d * (P.spin.y * P.hx.z - P.spin.z * P.hx.y - P.hv.x);
-        F.y = factor3 * P.hx.y / d
-            + P.td_orth_star / d * (hs.z * P.hx.x - hs.x * P.hx.z - P.hv.y)
-            + P.td_orth_planet / d * (P.spin.z * P.hx.x - P.spin.x * P.hx.z - P.hv.y);
-        F.z = factor3 * P.hx.z / d
-            + P.td_orth_star / d * (hs.x * P.hx.y - hs.y * P.hx.x - P.hv.z)
-            + P.td_orth_planet / d * (P.spin.x * P.hx.y - P.spin.y * P.hx.x - P.hv.z);
+        F.x = factor3 * x.x / d
+            + P.td_orth_star / d * (hs.y * x.z - hs.z * x.y - u.x)
+            + P.td_orth_planet / d * (P.spin.y * x.z - P.spin.z * x.y - u.x);
+        F.y = factor3 * x.y / d
+            + P.td_orth_star / d * (hs.z * x.x - hs.x * x.z - u.y)
+            + P.td_orth_planet / d * (P.spin.z * x.x - P.spin.x * x.z - u.y);
+        F.z = factor3 * x.z / d
+            + P.td_orth_star / d * (hs.x * x.y - hs.y * x.x - u.z)
+            + P.td_orth_planet / d * (P.spin.x * x.y - P.spin.y * x.x - u.z);
         const R factor1 = R(1.) / P.m;
         sum.x = sum.x + F.x; sum.y = sum.y + F.y; sum.z = sum.z + F.z;
         P.td_acc.x = factor1 * F.x; P.td_acc.y = factor1 * F.y; P.td_acc.z = factor1 * F.z;
@@ -649,8 +711,9 @@ template <class R> inline void flattening_radial(System<R> &S) {
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
-        P.rf_radial = R(-3.) / pw5(P.hdist) * (P.rf_fac_planet + P.rf_fac_star)
-            + R(15.) / pw7(P.hdist) * (P.rf_fac_star * P.rf_rs_star * P.rf_rs_star / H.spin2
+        if (!(P.roles & POS_ROLE_FLAT_ORBITING)) continue;
+        P.rf_radial = R(-3.) / pw5(P.rf_d) * (P.rf_fac_planet + P.rf_fac_star)
+            + R(15.) / pw7(P.rf_d) * (P.rf_fac_star * P.rf_rs_star * P.rf_rs_star / H.spin2
                                       + P.rf_fac_planet * P.rf_rs_planet * P.rf_rs_planet / P.spin2);
     }
 }
@@ -662,10 +725,11 @@ template <class R> inline void flattening_acceleration(System<R> &S) {
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
+        if (!(P.roles & POS_ROLE_FLAT_ORBITING)) continue;
         V3<R> F;
-        F.x = P.rf_radial * P.hx.x + P.rf_orth_planet * P.spin.x + P.rf_orth_star * H.spin.x;
-        F.y = P.rf_radial * P.hx.y + P.rf_orth_planet * P.spin.y + P.rf_orth_star * H.spin.y;
-        F.z = P.rf_radial * P.hx.z + P.rf_orth_planet * P.spin.z + P.rf_orth_star * H.spin.z;
+        F.x = P.rf_radial * P.rf_x.x + P.rf_orth_planet * P.spin.x + P.rf_orth_star * H.spin.x;
+        F.y = P.rf_radial * P.rf_x.y + P.rf_orth_planet * P.spin.y + P.rf_orth_star * H.spin.y;
+        F.z = P.rf_radial * P.rf_x.z + P.rf_orth_planet * P.spin.z + P.rf_orth_star * H.spin.z;
         sum.x = sum.x + F.x; sum.y = sum.y + F.y; sum.z = sum.z + F.z;
         const R factor1 = R(1.) / P.m;
         P.rf_acc.x = factor1 * F.x; P.rf_acc.y = factor1 * F.y; P.rf_acc.z = factor1 * F.z;
@@ -676,11 +740,12 @@ template <class R> inline void flattening_acceleration(System<R> &S) {
 }
 // constant_time_lag.rs:171-204 and tides/common.rs:223-261
 template <class R> inline V3<R> ctl_torque(const Body<R> &P, const V3<R> &ref_spin, const R &rs, const R &orth) {
-    const R d = P.hdist;
+    const R d = P.td_d;
+    const V3<R> &x = P.td_x, &u = P.td_v;
     V3<R> N;
-    N.x = orth * (d * ref_spin.x - rs * P.hx.x / d - R(1.0) / d * (P.hx.y * P.hv.z - P.hx.z * P.hv.y));
-    N.y = orth * (d * ref_spin.y - rs * P.hx.y / d - R(1.0) / d * (P.hx.z * P.hv.x - P.hx.x * P.hv.z));
-    N.z = orth * (d * ref_spin.z - rs * P.hx.z / d - R(1.0) / d * (P.hx.x * P.hv.y - P.hx.y * P.hv.x));
+    N.x = orth * (d * ref_spin.x - rs * x.x / d - R(1.0) / d * (x.y * u.z - x.z * u.y));
+    N.y = orth * (d * ref_spin.y - rs * x.y / d - R(1.0) / d * (x.z * u.x - x.x * u.z));
+    N.z = orth * (d * ref_spin.z - rs * x.z / d - R(1.0) / d * (x.x * u.y - x.y * u.x));
     return N;
 }
 template <class R> inline void tides_torques(System<R> &S) {
@@ -689,9 +754,12 @@ template <class R> inline void tides_torques(System<R> &S) {
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
+        if (!(P.roles & POS_ROLE_TIDES_ORBITING)) continue;
         V3<R> N = ctl_torque(P, P.spin, P.td_rs_planet, P.td_orth_planet);
         P.td_dLdt.x = factor * N.x; P.td_dLdt.y = factor * N.y; P.td_dLdt.z = factor * N.z;
     }
+    // the host's torque runs over every other body: the test of
+    // tides/common.rs:244-255 is on the host's effect, not the body's
     V3<R> sum = v3zero<R>();
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
@@ -703,10 +771,11 @@ template <class R> inline void tides_torques(System<R> &S) {
 }
 // oblate_spheroid.rs:62-81 and rotational_flattening/common.rs:165-201
 template <class R> inline V3<R> flat_torque(const Body<R> &P, const V3<R> &s, const R &orth) {
+    const V3<R> &x = P.rf_x;
     V3<R> N;
-    N.x = orth * (P.hx.y * s.z - P.hx.z * s.y);
-    N.y = orth * (P.hx.z * s.x - P.hx.x * s.z);
-    N.z = orth * (P.hx.x * s.y - P.hx.y * s.x);
+    N.x = orth * (x.y * s.z - x.z * s.y);
+    N.y = orth * (x.z * s.x - x.x * s.z);
+    N.z = orth * (x.x * s.y - x.y * s.x);
     return N;
 }
 template <class R> inline void flattening_torques(System<R> &S) {
@@ -715,6 +784,7 @@ template <class R> inline void flattening_torques(System<R> &S) {
     for (int i = 0; i < S.n; ++i) {
         if (i == S.host) continue;
         Body<R> &P = S.p[i];
+        if (!(P.roles & POS_ROLE_FLAT_ORBITING)) continue;
         V3<R> N = flat_torque(P, P.spin, P.rf_orth_planet);
         P.rf_dLdt.x = factor * N.x; P.rf_dLdt.y = factor * N.y; P.rf_dLdt.z = factor * N.z;
     }
@@ -1275,6 +1345,38 @@ template <class R> inline void stiefel_gs3(const R &beta, const R &x, R gs[4], u
     gs[3] = gs[3] * (x2 * x);
 }
 
+// Tangent-linear runs only (R = Dual<K>): the derivative of the bisection's
+// root.  The bisection (whfast.rs:776-808) finds the root of the universal
+// Kepler equation s(x) = r0 x + eta0 G2(x) + zeta0 G3(x) - dt to 1e-15, so
+// the step's map follows that root; differentiating the bisection iterates
+// instead gives the derivatives of the bracket's end points.  The root's
+// derivative is dx = -s_p / s_x (implicit function theorem, with
+// s_x = r0 + eta0 G1 + zeta0 G2), and the Stiefel functions of the last
+// evaluation take their derivatives from it.  Values are not touched.
+template <class R> struct IsDual { static constexpr bool value = false; };
+template <int K> struct IsDual<Dual<K>> { static constexpr bool value = true; };
+template <class R>
+inline void bisection_root_derivative(const R &beta, const R &r0, const R &eta0, const R &zeta0, const R &dt,
+                                      R &x_eval, R &x, R gs[4], uint32_t *status) {
+    if constexpr (IsDual<R>::value) {
+        const R xc(val(x_eval));   // the last evaluated point, no derivative
+        R g[4];
+        uint32_t st = 0;
+        stiefel_gs3(beta, xc, g, &st);
+        const R sres = r0 * xc + eta0 * g[2] + zeta0 * g[3] - dt;
+        const double sx = val(r0) + val(eta0) * val(g[1]) + val(zeta0) * val(g[2]);
+        R xd(val(x_eval));
+        for (int k = 0; k < (int)(sizeof(xd.d) / sizeof(xd.d[0])); ++k) xd.d[k] = -sres.d[k] / sx;
+        x_eval = xd;
+        stiefel_gs3(beta, x_eval, gs, status);   // same values as the last evaluation
+        R xf(val(x));
+        for (int k = 0; k < (int)(sizeof(xf.d) / sizeof(xf.d[0])); ++k) xf.d[k] = xd.d[k];
+        x = xf;
+    } else {
+        (void)beta; (void)r0; (void)eta0; (void)zeta0; (void)dt; (void)x_eval; (void)x; (void)gs; (void)status;
+    }
+}
+
 // whfast.rs:676-833
 template <class R> inline void kepler_individual_step(System<R> &S, int i, const R &mass_g, const R &dt) {
     Alt<R> &A = S.alt[i];
@@ -1358,14 +1460,17 @@ template <class R> inline void kepler_individual_step(System<R> &S, int i, const
         }
         x = (x_max + x_min) / R(2.);
         int it = 0;
+        R x_eval = x;
         for (;;) {
             if (++it > BISECTION_CAP) { S.status |= POS_STATUS_ITERATION_CAP; break; }
+            x_eval = x;
             stiefel_gs3(beta, x, gs, &S.status);
             const R s = r0 * x + eta0 * gs[2] + zeta0 * gs[3] - dt;
             if (val(s) >= 0.) x_max = x; else x_min = x;
             x = (x_max + x_min) / R(2.);
             if (rfabs(val(x_max) - val(x_min)) / val(x_max) <= 1e-15) break;
         }
+        bisection_root_derivative(beta, r0, eta0, zeta0, dt, x_eval, x, gs, &S.status);
         eta0_gs1_zeta0_gs2 = eta0 * gs[1] + zeta0 * gs[2];
         ri = R(1.) / (r0 + eta0_gs1_zeta0_gs2);
     }

@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAX_BODIES = 16
 
 OK = 0
@@ -123,6 +123,7 @@ class EnsembleDesc(C.Structure):
         ("evolver_inverse_tidal_q_factor", _pd),
         ("disk_properties", _pd),
         ("pair_dissipation_factor", _pd),
+        ("roche_radius", _pd),
     ]
 
 
@@ -145,6 +146,7 @@ class State(C.Structure):
         ("evolver_left_index", _pi32),
         ("timestep_warning", _pi32),
         ("pair_dissipation_factor", _pd),
+        ("roche_radius", _pd),
     ]
 
 
@@ -157,6 +159,11 @@ class Options(C.Structure):
         ("use_graph", C.c_int32),
         ("reserved", C.c_int32 * 3),
     ]
+
+
+class LaunchStats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("kernel_launches", C.c_int64), ("graph_replays", C.c_int64),
+                ("launches_per_replay", C.c_int64), ("last_operator_kernel_ms", C.c_double)]
 
 
 class HistoryRecord(C.Structure):
@@ -203,6 +210,7 @@ STATE_FIELDS = {
     "evolver_left_index": (1, np.int32),
     "timestep_warning": (0, np.int32),
     "pair_dissipation_factor": (1, np.float64),
+    "roche_radius": (MAX_BODIES, np.float64),
 }
 
 _CTYPE = {np.float64: C.c_double, np.int64: C.c_int64, np.uint32: C.c_uint32, np.int32: C.c_int32}
@@ -220,7 +228,7 @@ def alloc_state(n_systems: int, n_bodies: int, fields=None):
         elif w == 1:
             a = np.zeros(n_bodies, dtype=dt)
         else:
-            a = np.zeros((n_bodies, 3), dtype=dt)
+            a = np.zeros((n_bodies, w), dtype=dt)
         arrays[name] = a
         setattr(st, name, ptr(a, _CTYPE[dt]))
     return st, arrays
@@ -277,6 +285,8 @@ def _declare(lib):
     lib.pos_get_stream.restype = C.c_int32
     lib.pos_get_options.argtypes = [C.c_void_p, C.POINTER(Options)]
     lib.pos_get_options.restype = C.c_int32
+    lib.pos_get_launch_stats.argtypes = [C.c_void_p, C.POINTER(LaunchStats)]
+    lib.pos_get_launch_stats.restype = C.c_int32
     lib.pos_tune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Options)]
     lib.pos_tune.restype = C.c_int32
     lib.pos_gravity_acceleration.argtypes = [C.c_void_p, _pd, _pu32]
@@ -298,7 +308,8 @@ def _declare(lib):
 EXPORTED_SYMBOLS = (
     "pos_abi_version", "pos_last_error", "pos_create", "pos_step", "pos_synchronize",
     "pos_get_state", "pos_set_state", "pos_set_time_limit", "pos_set_historic_snapshot_period",
-    "pos_get_counts", "pos_get_stream", "pos_get_options", "pos_tune", "pos_gravity_acceleration",
+    "pos_get_counts", "pos_get_stream", "pos_get_options", "pos_get_launch_stats", "pos_tune",
+    "pos_gravity_acceleration",
     "pos_additional_effects", "pos_history_enable", "pos_history_drain", "pos_adjoint_record", "pos_adjoint",
     "pos_destroy",
 )

@@ -37,9 +37,10 @@ BODY_FIELDS = {
     "wind_rotation_saturation": (np.float64, 1), "velocity_error": (np.float64, 3),
     "angular_momentum_error": (np.float64, 3), "evolution_type": (np.int32, 1),
     "evolver_left_index": (np.int32, 1), "pair_dissipation_factor": (np.float64, 1),
+    "roche_radius": (np.float64, abi.MAX_BODIES),
 }
 # fill value of a per-body field a spec does not carry (NaN: no map entry)
-BODY_FILL = {"pair_dissipation_factor": np.nan}
+BODY_FILL = {"pair_dissipation_factor": np.nan, "roche_radius": np.nan}
 MAX_PARTICLES = 10   # src/constants.rs:3, the key base of the pair-dependent factor map
 TABLE_FIELDS = ("evolver_time", "evolver_radius", "evolver_radius_of_gyration_2", "evolver_love_number",
                 "evolver_inverse_tidal_q_factor")
@@ -106,7 +107,7 @@ class EnsembleSpec:
             filled = []
             for s, p in zip(specs, parts):
                 if p is None:
-                    p = np.full((s.n_bodies, 3) if w == 3 else s.n_bodies, BODY_FILL.get(k, 0), dtype=dt)
+                    p = np.full((s.n_bodies, w) if w > 1 else s.n_bodies, BODY_FILL.get(k, 0), dtype=dt)
                 filled.append(p)
             out.body[k] = np.concatenate(filled).astype(dt)
         if any(s.evolver_offset is not None for s in specs):
@@ -337,9 +338,16 @@ def read_case(path_or_dict) -> EnsembleSpec:
         if dep == ids[host] and owner in ids and ids.index(owner) != host:
             pairs[ids.index(owner)] = float(value)
     body["pair_dissipation_factor"] = pairs
+    # Roche radii (universe.rs:177-196): row i of the n x n block holds body
+    # i's pairs; a resumed case keeps the values computed at t = 0
+    roche = np.zeros((n, abi.MAX_BODIES))
+    flat = u.get("roche_radiuses") or []
+    if len(flat) >= n * n:
+        roche[:, :n] = np.asarray(flat[:n * n], dtype=np.float64).reshape(n, n)
+    body["roche_radius"] = roche
     for k, (dt, w) in BODY_FIELDS.items():
         a = np.asarray(body[k], dtype=dt)
-        spec.body[k] = a.reshape(n, 3) if w == 3 else a
+        spec.body[k] = a.reshape(n, w) if w > 1 else a
     spec.evolver_offset = np.asarray(eo, dtype=np.int32)
     spec.tables = {k: (np.concatenate(v) if v else np.zeros(0)).astype(np.float64) for k, v in tabs.items()}
     return spec

@@ -74,12 +74,21 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
     if ((B.evo == POS_EVO_LECONTE_CHABRIER2013 || B.evo == POS_EVO_LECONTE_CHABRIER2013_DYN) &&
         (roles & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_TIDES_ORBITING)))
         f |= F_LC_LOVE;
+    if (valid && k != B.host) {
+        if ((f & F_TIDES) && (roles & POS_ROLE_TIDES_ORBITING)) f |= F_TIDES_ROLE;
+        if ((f & F_RF) && (roles & POS_ROLE_FLAT_ORBITING)) f |= F_RF_ROLE;
+        if ((f & F_GR) && (roles & POS_ROLE_GR_ORBITING)) f |= F_GR_ROLE;
+    }
     if constexpr ((V & VAR_FAST) != 0) {   // the launch holds only systems of the fixed effect set
-        f = (valid ? F_VALID : 0u) | ((valid && k == B.host) ? F_HOST : 0u) | F_TIDES | F_RF | F_GR | F_SPIN;
+        const bool orb = valid && k != B.host;   // every other body has every orbiting role there
+        f = (valid ? F_VALID : 0u) | ((valid && k == B.host) ? F_HOST : 0u) | F_TIDES | F_RF | F_GR | F_SPIN |
+            (orb ? (F_TIDES_ROLE | F_RF_ROLE | F_GR_ROLE) : 0u);
         B.coord = POS_COORD_DEMOCRATIC_HELIOCENTRIC;
     }
     B.flags = f;
     B.dt = E.dt[s];
+    B.hist_period = E.hist_period[s];
+    B.time_limit = E.time_limit[s];
     B.t = E.t[s];
     B.last_hist = E.last_hist[s];
     B.iteration = E.iteration[s];
@@ -348,10 +357,12 @@ struct pos_ens {
     hipGraph_t graph = nullptr;
     int graph_launches = 0;
     // per system / per body kinds in device order (adjoint support checks)
-    std::vector<uint32_t> eff_dev;
-    std::vector<int> coord_dev, evo_dev;
+    std::vector<uint32_t> eff_dev, roles_dev;
+    std::vector<int> coord_dev, evo_dev, host_dev;
     std::vector<double> mass_dev, sigma_dev, dt_dev;   // constants the history records carry
     int hist_cap = 0;
+    pos_launch_stats stats{};
+    hipEvent_t op_ev[2] = {nullptr, nullptr};   // around the kernels of an operator call
     // discrete adjoint: checkpoints of the recorded run and the segment tape
     struct Adjoint {
         bool recorded = false;
@@ -568,25 +579,56 @@ int build_launch(pos_ens *h, const pos_options &o) {
     return POS_OK;
 }
 
+// consider_effects after Universe::new switched off what no body carries
+// (disable_unnecessary_effects, universe.rs:699-764): tides, flattening, GR and
+// the disk without a central body, wind without a wind body, evolution without
+// an evolving body; GR also when its central body's implementation is Disabled.
+uint32_t effective_effects(const pos_ensemble_desc *d, int s) {
+    uint32_t eff = d->effects[s];
+    if (d->gr_implementation[s] == POS_GR_DISABLED) eff &= ~POS_EFFECT_GENERAL_RELATIVITY;
+    uint32_t any = 0;
+    bool evolving = false;
+    for (int b = d->body_offset[s]; b < d->body_offset[s + 1]; ++b) {
+        any |= d->roles[b];
+        if (d->evolution_type && d->evolution_type[b] != POS_EVO_NON_EVOLVING) evolving = true;
+    }
+    if (!(any & POS_ROLE_TIDES_CENTRAL)) eff &= ~POS_EFFECT_TIDES;
+    if (!(any & POS_ROLE_FLAT_CENTRAL)) eff &= ~POS_EFFECT_ROTATIONAL_FLATTENING;
+    if (!(any & POS_ROLE_GR_CENTRAL)) eff &= ~POS_EFFECT_GENERAL_RELATIVITY;
+    if (!(any & POS_ROLE_DISK_CENTRAL)) eff &= ~POS_EFFECT_DISK;
+    if (!(any & POS_ROLE_WIND)) eff &= ~POS_EFFECT_WIND;
+    if (!evolving) eff &= ~POS_EFFECT_EVOLUTION;
+    return eff;
+}
+
+// Every body of system s but its host is an orbiting body of tides,
+// flattening and general relativity.
+bool every_body_orbits(const pos_ensemble_desc *d, int s, int host) {
+    constexpr uint32_t ALL = POS_ROLE_TIDES_ORBITING | POS_ROLE_FLAT_ORBITING | POS_ROLE_GR_ORBITING;
+    const int b0 = d->body_offset[s], n = d->body_offset[s + 1] - b0;
+    for (int i = 0; i < n; ++i)
+        if (i != host && (d->roles[b0 + i] & ALL) != ALL) return false;
+    return true;
+}
+
 // Kernel variant a system's steps are compiled for: its GR implementation
 // (GRV_KIDDER also when GR is off) plus VAR_EXT with the disk effect or an
 // evolving-dissipation host.
 int system_variant(const pos_ensemble_desc *d, int s, int host) {
-    const uint32_t effects = d->effects[s];
+    const uint32_t effects = effective_effects(d, s);
     const int implementation = d->gr_implementation[s];
     int v = GRV_KIDDER;
     if (effects & POS_EFFECT_GENERAL_RELATIVITY) {
         if (implementation == POS_GR_ANDERSON1975) v = GRV_ANDERSON;
         if (implementation == POS_GR_NEWHALL1983) v = GRV_NEWHALL;
     }
-    (void)host;
     bool dyn = false;   // an evolving-dissipation body (host or orbiting)
     if (d->evolution_type)
         for (int b = d->body_offset[s]; b < d->body_offset[s + 1]; ++b) dyn = dyn || evo_dynamical(d->evolution_type[b]);
     if ((effects & POS_EFFECT_DISK) || dyn) v |= VAR_EXT;
     constexpr uint32_t FAST_EFFECTS = POS_EFFECT_TIDES | POS_EFFECT_ROTATIONAL_FLATTENING | POS_EFFECT_GENERAL_RELATIVITY;
     if (v == GRV_KIDDER && effects == FAST_EFFECTS && implementation == POS_GR_KIDDER1995 &&
-        d->coordinates[s] == POS_COORD_DEMOCRATIC_HELIOCENTRIC)
+        d->coordinates[s] == POS_COORD_DEMOCRATIC_HELIOCENTRIC && every_body_orbits(d, s, host))
         v |= VAR_FAST;
     return v;
 }
@@ -623,34 +665,43 @@ int validate(const pos_ensemble_desc *d) {
         if ((eff & POS_EFFECT_GENERAL_RELATIVITY) &&
             (d->gr_implementation[s] < POS_GR_KIDDER1995 || d->gr_implementation[s] > POS_GR_DISABLED))
             return fail(POS_E_INVALID, where + "bad general relativity implementation");
-        const bool gr_on = (eff & POS_EFFECT_GENERAL_RELATIVITY) && d->gr_implementation[s] != POS_GR_DISABLED;
         if (!(d->time_step[s] > 0.)) return fail(POS_E_INVALID, where + "time_step must be positive");
         int mm = -1;
         double mmax = 0.;
         for (int i = 0; i < n; ++i)
             if (d->mass[b0 + i] > mmax) { mmax = d->mass[b0 + i]; mm = i; }
         if (mm < 0) return fail(POS_E_INVALID, where + "no body has a positive mass");
-        const uint32_t hr = d->roles[b0 + mm];
-        if ((eff & POS_EFFECT_TIDES) && !(hr & POS_ROLE_TIDES_CENTRAL))
-            return fail(POS_E_UNSUPPORTED, where + "the tidal host must be the most massive body");
-        if ((eff & POS_EFFECT_ROTATIONAL_FLATTENING) && !(hr & POS_ROLE_FLAT_CENTRAL))
-            return fail(POS_E_UNSUPPORTED, where + "the flattening host must be the most massive body");
-        if (gr_on && !(hr & POS_ROLE_GR_CENTRAL))
-            return fail(POS_E_UNSUPPORTED, where + "the GR host must be the most massive body");
-        if ((eff & POS_EFFECT_DISK) && !(hr & POS_ROLE_DISK_CENTRAL))
+        // the central body of each effect (universe.rs:699-764, 900-1006): at
+        // most one per effect; an effect without one is switched off
+        // (effective_effects); the reference panics on the conditions below
+        const uint32_t central[4] = {POS_ROLE_TIDES_CENTRAL, POS_ROLE_FLAT_CENTRAL, POS_ROLE_GR_CENTRAL, POS_ROLE_DISK_CENTRAL};
+        const char *names[4] = {"tidal", "rotational flattening", "general relativity", "disk"};
+        int host_of[4] = {-1, -1, -1, -1};
+        for (int e = 0; e < 4; ++e)
+            for (int i = 0; i < n; ++i)
+                if (d->roles[b0 + i] & central[e]) {
+                    if (host_of[e] >= 0) return fail(POS_E_INVALID, where + "only one central body is allowed for " + names[e] + " effects");
+                    host_of[e] = i;
+                }
+        const uint32_t on = effective_effects(d, s);
+        const bool tides = on & POS_EFFECT_TIDES, rf = on & POS_EFFECT_ROTATIONAL_FLATTENING;
+        if (tides && rf && host_of[0] != host_of[1])
+            return fail(POS_E_INVALID, where + "the central body for tidal & rotational flattening effects needs to be the same");
+        if ((on & POS_EFFECT_GENERAL_RELATIVITY) && host_of[2] != mm)
+            return fail(POS_E_INVALID, where + "the central body for general relativity should be the most massive one");
+        // a tidal / flattening / disk host other than the most massive body
+        // (hosts.most_massive.all false, universe.rs:386-419) is not built
+        if ((tides || rf) && host_of[tides ? 0 : 1] != mm)
+            return fail(POS_E_UNSUPPORTED, where + "the tidal and flattening host must be the most massive body");
+        if ((on & POS_EFFECT_DISK) && host_of[3] != mm)
             return fail(POS_E_UNSUPPORTED, where + "the disk host must be the most massive body");
-        for (int i = 0; i < n; ++i) {
-            if (i == mm) continue;
-            const uint32_t r = d->roles[b0 + i];
-            if (r & (POS_ROLE_TIDES_CENTRAL | POS_ROLE_FLAT_CENTRAL | POS_ROLE_GR_CENTRAL | POS_ROLE_DISK_CENTRAL))
-                return fail(POS_E_UNSUPPORTED, where + "only one central body per system");
-            if ((eff & POS_EFFECT_TIDES) && !(r & POS_ROLE_TIDES_ORBITING))
-                return fail(POS_E_UNSUPPORTED, where + "with tides on every other body must be a tidal orbiting body");
-            if ((eff & POS_EFFECT_ROTATIONAL_FLATTENING) && !(r & POS_ROLE_FLAT_ORBITING))
-                return fail(POS_E_UNSUPPORTED, where + "with flattening on every other body must be an orbiting body");
-            if (gr_on && !(r & POS_ROLE_GR_ORBITING))
-                return fail(POS_E_UNSUPPORTED, where + "with GR on every other body must be an orbiting body");
-        }
+        // Anderson 1975 / Newhall 1983 read the GR coordinates of every body,
+        // which the reference leaves stale for a body whose GR is Disabled
+        if ((on & POS_EFFECT_GENERAL_RELATIVITY) && d->gr_implementation[s] != POS_GR_KIDDER1995)
+            for (int i = 0; i < n; ++i)
+                if (i != mm && !(d->roles[b0 + i] & POS_ROLE_GR_ORBITING))
+                    return fail(POS_E_UNSUPPORTED, where + "Anderson1975 and Newhall1983 need every other body to be a "
+                                                           "general relativity orbiting body");
         const double last = d->last_historic_snapshot_time ? d->last_historic_snapshot_time[s] : -1.;
         if (last >= 0. && d->heliocentric_velocity) {
             const double *hv = d->heliocentric_velocity + 3 * (b0 + mm);
@@ -718,15 +769,74 @@ int adjoint_copy_state(pos_ens *h, char *slot, bool save) {
 
 constexpr int GRAPH_LAUNCHES = 4;   // kernel launches per captured graph
 
+void drop_graph(pos_ens *h) {
+    if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    h->graph_launches = 0;
+}
+
+// The device view of the ensemble with historic snapshots switched off: steps
+// that are not part of the caller's run (autotuner probes, the adjoint's
+// segment recomputation) must not append records.
+DevEnsemble without_history(const DevEnsemble &E) {
+    DevEnsemble r = E;
+    r.hist = nullptr;
+    r.hist_count = nullptr;
+    r.hist_cap = 0;
+    return r;
+}
+
+// Detaches the history buffer from the handle for the lifetime of the guard
+// (every launch made meanwhile, graph captures included, sees none) and
+// reattaches it on release; a graph captured while detached is dropped then.
+struct HistoryDetach {
+    pos_ens *h;
+    double *hist;
+    int *count;
+    int cap;
+    bool active = true;
+    explicit HistoryDetach(pos_ens *h_) : h(h_), hist(h_->E.hist), count(h_->E.hist_count), cap(h_->E.hist_cap) {
+        h->E = without_history(h->E);
+    }
+    void release() {
+        if (!active) return;
+        active = false;
+        h->E.hist = hist;
+        h->E.hist_count = count;
+        h->E.hist_cap = cap;
+        if (hist) drop_graph(h);
+    }
+    ~HistoryDetach() { release(); }
+};
+
+// Two timing events, destroyed with the guard.
+struct EventPair {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    hipError_t create() {
+        for (auto &x : e) {
+            const hipError_t r = hipEventCreate(&x);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+    ~EventPair() {
+        for (auto &x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
 template <int V> void launch_steps_variant(pos_ens *h, int grid, int n_steps, int w0, int w1) {
     hipLaunchKernelGGL(whfast_steps_kernel<V>, dim3(grid), dim3(64), 0, h->stream, h->E, n_steps, w0, w1);
 }
 
 // One launch per kernel variant present; the persistent grid size caps each.
-int launch_steps(pos_ens *h, int n_steps) {
+// Returns the number of launches through *launches (optional).
+int launch_steps(pos_ens *h, int n_steps, int *launches = nullptr) {
+    if (launches) *launches = 0;
     for (int v = 0; v < N_VARIANTS; ++v) {
         const int w0 = h->variant_waves[v][0], w1 = h->variant_waves[v][1];
         if (w1 <= w0) continue;
+        if (launches) ++*launches;
         const int grid = std::min(h->grid, w1 - w0);
         switch (v) {
             case 0: launch_steps_variant<0>(h, grid, n_steps, w0, w1); break;
@@ -750,12 +860,23 @@ int ensure_graph(pos_ens *h) {
     if (h->graph_exec) return POS_OK;
     HIP_TRY(hipStreamSynchronize(h->stream));   // capture starts on an idle stream
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int crc = POS_OK;
-    for (int i = 0; i < GRAPH_LAUNCHES && crc == POS_OK; ++i) crc = launch_steps(h, h->steps_per_launch);
+    int crc = POS_OK, per = 0, total = 0;
+    for (int i = 0; i < GRAPH_LAUNCHES && crc == POS_OK; ++i) {
+        crc = launch_steps(h, h->steps_per_launch, &per);
+        total += per;
+    }
     HIP_TRY(hipStreamEndCapture(h->stream, &h->graph));
     if (crc) return crc;
     HIP_TRY(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
-    h->graph_launches = GRAPH_LAUNCHES;
+    h->graph_launches = total;
+    return POS_OK;
+}
+
+// Device time between the two operator events (both recorded and completed).
+int record_operator_time(pos_ens *h) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->op_ev[0], h->op_ev[1]));
+    h->stats.last_operator_kernel_ms = ms;
     return POS_OK;
 }
 
@@ -784,6 +905,10 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return fail(POS_E_HIP, "hipStreamCreate failed");
+    }
+    if (hipEventCreate(&h->op_ev[0]) != hipSuccess || hipEventCreate(&h->op_ev[1]) != hipSuccess) {
+        pos_destroy(h);
+        return fail(POS_E_HIP, "hipEventCreate failed");
     }
     const int ns = d->n_systems;
     const int nb = d->body_offset[ns];
@@ -828,8 +953,7 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
             if (d->mass[b0 + i] > mmax) { mmax = d->mass[b0 + i]; mm = i; }
         host_idx[ds] = mm;
         coord[ds] = d->coordinates[s];
-        uint32_t e = d->effects[s];
-        if (d->gr_implementation[s] == POS_GR_DISABLED) e &= ~POS_EFFECT_GENERAL_RELATIVITY;
+        const uint32_t e = effective_effects(d, s);
         eff[ds] = e | ((uint32_t)h->variant_dev[ds] << EFF_VAR_SHIFT);
         dt[ds] = d->time_step[s];
         tl[ds] = d->time_limit[s];
@@ -954,7 +1078,10 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
         hipLaunchKernelGGL(init_kernel, dim3(h->E.n_waves), dim3(64), 0, h->stream, h->E);
         if (hipGetLastError() != hipSuccess) { pos_destroy(h); return fail(POS_E_HIP, "init kernel launch failed"); }
     }
-    // Roche radii from the (possibly evolved) radii (universe.rs:177-196)
+    // Roche radii (universe.rs:177-196): initialize_physical_values computes
+    // them once, at t = 0, from the radii evolved to t = 0; a resumed system
+    // keeps the values its snapshot carries (or, without them, gets them from
+    // its current radii)
     {
         std::vector<double> Rnow;
         if ((rc = download(h, Rnow, d_R, nb))) { pos_destroy(h); return rc; }
@@ -962,16 +1089,23 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
         std::vector<double> roche((size_t)nb * POS_MAX_BODIES, 0.);
         for (int ds = 0; ds < ns; ++ds) {
             const int b0 = h->body_offset_dev[ds], n = h->n_per_sys_dev[ds];
+            const bool resumed = t[ds] != 0. && d->roche_radius;
             for (int i = 0; i < n; ++i)
-                for (int j = 0; j < n; ++j)
-                    if (i != j)
-                        roche[(size_t)(b0 + i) * POS_MAX_BODIES + j] = host_roche(m[b0 + i], m[b0 + j], Rnow[b0 + i], Rnow[b0 + j]);
+                for (int j = 0; j < n; ++j) {
+                    if (i == j) continue;
+                    const double carried =
+                        resumed ? d->roche_radius[(size_t)h->body_perm[b0 + i] * POS_MAX_BODIES + j] : std::nan("");
+                    roche[(size_t)(b0 + i) * POS_MAX_BODIES + j] =
+                        std::isnan(carried) ? host_roche(m[b0 + i], m[b0 + j], Rnow[b0 + i], Rnow[b0 + j]) : carried;
+                }
         }
         if ((rc = upload(h, d_roche, roche))) { pos_destroy(h); return rc; }
     }
     if ((rc = build_launch(h, o))) { pos_destroy(h); return rc; }
     if (hipStreamSynchronize(h->stream) != hipSuccess) { pos_destroy(h); return fail(POS_E_HIP, "synchronize failed"); }
     h->eff_dev = eff;
+    h->roles_dev = roles;
+    h->host_dev = host_idx;
     h->coord_dev = coord;
     h->evo_dev = evo_type;
     h->mass_dev = m;
@@ -993,18 +1127,26 @@ int32_t pos_step(pos_ens *h, int64_t n_steps) {
     const int K = h->steps_per_launch;
     int64_t full = n_steps / K;
     const int rest = (int)(n_steps - full * K);
+    h->stats.steps += n_steps;
     if (h->opts.use_graph && full >= GRAPH_LAUNCHES) {
         int grc = ensure_graph(h);
         if (grc) return grc;
         while (full >= GRAPH_LAUNCHES) {
             HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
             full -= GRAPH_LAUNCHES;
+            h->stats.graph_replays += 1;
+            h->stats.launches_per_replay = h->graph_launches;
         }
     }
-    int rc;
-    for (int64_t i = 0; i < full; ++i)
-        if ((rc = launch_steps(h, K))) return rc;
-    if (rest > 0 && (rc = launch_steps(h, rest))) return rc;
+    int rc, nl = 0;
+    for (int64_t i = 0; i < full; ++i) {
+        if ((rc = launch_steps(h, K, &nl))) return rc;
+        h->stats.kernel_launches += nl;
+    }
+    if (rest > 0) {
+        if ((rc = launch_steps(h, rest, &nl))) return rc;
+        h->stats.kernel_launches += nl;
+    }
     return POS_OK;
 }
 
@@ -1062,6 +1204,14 @@ int32_t pos_get_state(pos_ens *h, pos_state *o) {
         for (int ds = 0; ds < ns; ++ds) o->timestep_warning[h->sys_perm[ds]] = tw[ds];
     }
     if ((rc = vec1(o->pair_dissipation_factor, E.pair_ph))) return rc;
+    if (o->roche_radius) {   // rows move with their bodies; columns index within the system
+        std::vector<double> rr;
+        if ((rc = download(h, rr, E.roche, (size_t)nb * POS_MAX_BODIES))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (int db = 0; db < nb; ++db)
+            std::memcpy(o->roche_radius + (size_t)h->body_perm[db] * POS_MAX_BODIES, rr.data() + (size_t)db * POS_MAX_BODIES,
+                        POS_MAX_BODIES * sizeof(double));
+    }
     return POS_OK;
 }
 
@@ -1114,6 +1264,14 @@ int32_t pos_set_state(pos_ens *h, const pos_state *in) {
         HIP_TRY(hipStreamSynchronize(h->stream));
     }
     if ((rc = vec1(in->pair_dissipation_factor, E.pair_ph))) return rc;
+    if (in->roche_radius) {
+        std::vector<double> rr((size_t)nb * POS_MAX_BODIES);
+        for (int db = 0; db < nb; ++db)
+            std::memcpy(rr.data() + (size_t)db * POS_MAX_BODIES, in->roche_radius + (size_t)h->body_perm[db] * POS_MAX_BODIES,
+                        POS_MAX_BODIES * sizeof(double));
+        if ((rc = upload(h, const_cast<double *>(E.roche), rr))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
     return POS_OK;
 }
 
@@ -1179,11 +1337,14 @@ int32_t pos_gravity_acceleration(pos_ens *h, double *out_acc, uint32_t *out_stat
         return rc;
     double *d_out = h->d_op_a;
     uint32_t *d_st = h->d_op_st;
+    HIP_TRY(hipEventRecord(h->op_ev[0], h->stream));
     hipLaunchKernelGGL(gravity_kernel, dim3(h->E.n_waves), dim3(64), 0, h->stream, h->E, d_out, d_st);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(h->op_ev[1], h->stream));
     std::vector<uint32_t> st;
     if ((rc = download_bodies(h, out_acc, d_out, 3)) || (rc = download(h, st, d_st, ns))) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
+    if ((rc = record_operator_time(h))) return rc;
     if (out_status)
         for (int ds = 0; ds < ns; ++ds) out_status[h->sys_perm[ds]] |= st[ds];
     return POS_OK;
@@ -1199,6 +1360,7 @@ int32_t pos_additional_effects(pos_ens *h, double *out_acc, double *out_dLdt) {
                        (rc = dalloc(h, &h->d_op_st, h->n_systems))))
         return rc;
     double *d_a = h->d_op_a, *d_l = h->d_op_l;
+    HIP_TRY(hipEventRecord(h->op_ev[0], h->stream));
     for (int v = 0; v < N_VARIANTS; ++v) {
         const int w0 = h->variant_waves[v][0], w1 = h->variant_waves[v][1];
         if (w1 <= w0) continue;
@@ -1215,7 +1377,14 @@ int32_t pos_additional_effects(pos_ens *h, double *out_acc, double *out_dLdt) {
         }
         HIP_TRY(hipGetLastError());
     }
+    HIP_TRY(hipEventRecord(h->op_ev[1], h->stream));
     if ((rc = download_bodies(h, out_acc, d_a, 3)) || (rc = download_bodies(h, out_dLdt, d_l, 3))) return rc;
+    return record_operator_time(h);
+}
+
+int32_t pos_get_launch_stats(const pos_ens *h, pos_launch_stats *out) {
+    if (!h || !out) return fail(POS_E_INVALID, "NULL argument");
+    *out = h->stats;
     return POS_OK;
 }
 
@@ -1232,6 +1401,10 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
     std::vector<uint32_t> sst(ns);
     std::vector<int> stw(ns), sleft(nb);
     int rc;
+    // the probes are not part of the caller's run: no historic records, and
+    // the launch counters are restored at the end
+    HistoryDetach detached(h);
+    const pos_launch_stats stats0 = h->stats;
     const DevEnsemble &E = h->E;
     if ((rc = download(h, sx, E.x, 3 * (size_t)nb)) || (rc = download(h, sv, E.v, 3 * (size_t)nb)) ||
         (rc = download(h, sL, E.L, 3 * (size_t)nb)) || (rc = download(h, ss, E.spin, 3 * (size_t)nb)) ||
@@ -1276,9 +1449,9 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
     const int keep_graph = h->opts.use_graph;
     pos_options best = h->opts;
     double best_t = std::numeric_limits<double>::infinity();
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    EventPair ev;
+    HIP_TRY(ev.create());
+    const hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     for (int g : groups) {
         for (int K : ks) {
             pos_options o{};
@@ -1326,8 +1499,8 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
     } else {
         best.use_graph = 0;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    detached.release();
+    h->stats = stats0;
     if ((rc = build_launch(h, best))) return rc;
     if (best.use_graph && (rc = ensure_graph(h))) return rc;   // captured now, not in the caller's first steps
     if (chosen) *chosen = h->opts;
@@ -1444,6 +1617,16 @@ int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
         for (int b = h->body_offset_dev[ds]; b < h->body_offset_dev[ds + 1]; ++b)
             if (h->evo_dev[b] != POS_EVO_NON_EVOLVING)
                 return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support evolving bodies");
+        const uint32_t eff = h->eff_dev[ds];
+        for (int b = h->body_offset_dev[ds]; b < h->body_offset_dev[ds + 1]; ++b) {
+            if (b - h->body_offset_dev[ds] == h->host_dev[ds]) continue;
+            const uint32_t r = h->roles_dev[b];
+            if (((eff & POS_EFFECT_TIDES) && !(r & POS_ROLE_TIDES_ORBITING)) ||
+                ((eff & POS_EFFECT_ROTATIONAL_FLATTENING) && !(r & POS_ROLE_FLAT_ORBITING)) ||
+                ((eff & POS_EFFECT_GENERAL_RELATIVITY) && !(r & POS_ROLE_GR_ORBITING)))
+                return fail(POS_E_UNSUPPORTED, where + "the adjoint needs every body but the host to be an orbiting body "
+                                                       "of each effect that is on");
+        }
     }
     HIP_TRY(hipSetDevice(h->device));
     pos_ens::Adjoint &A = h->adj;
@@ -1550,10 +1733,12 @@ int32_t pos_adjoint(pos_ens *h, const pos_adjoint_seed *seed, pos_adjoint_grad *
             if (w1 <= w0) continue;
             const dim3 grid(std::min(h->grid, w1 - w0));
             const TapeRef tape{A.tape[buf], nb};
+            // the recomputed steps were recorded once already: no historic records
+            const DevEnsemble Er = without_history(h->E);
             if (v == GRV_KIDDER)
-                hipLaunchKernelGGL(whfast_record_kernel<GRV_KIDDER>, grid, dim3(64), 0, h->stream, h->E, seg_len(s), tape, w0, w1);
+                hipLaunchKernelGGL(whfast_record_kernel<GRV_KIDDER>, grid, dim3(64), 0, h->stream, Er, seg_len(s), tape, w0, w1);
             else if (v == (GRV_KIDDER | VAR_FAST))
-                hipLaunchKernelGGL(whfast_record_kernel<GRV_KIDDER | VAR_FAST>, grid, dim3(64), 0, h->stream, h->E, seg_len(s), tape,
+                hipLaunchKernelGGL(whfast_record_kernel<GRV_KIDDER | VAR_FAST>, grid, dim3(64), 0, h->stream, Er, seg_len(s), tape,
                                    w0, w1);
             else
                 return fail(POS_E_INVALID, "internal error: kernel variant without an adjoint");
@@ -1609,6 +1794,8 @@ void pos_destroy(pos_ens *h) {
     if (h->adj.sweep) (void)hipStreamSynchronize(h->adj.sweep);
     adjoint_release(h);
     for (void *p : h->allocs) (void)hipFree(p);
+    for (hipEvent_t e : h->op_ev)
+        if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }

@@ -162,7 +162,24 @@ struct WaveLds {
     double xb[NX][WAVE];
 };
 
-__device__ __forceinline__ void wsync() { __syncthreads(); }
+// Every kernel that uses the exchange area runs one wavefront per workgroup
+// (__launch_bounds__(64)), so the lanes that publish and the lanes that read
+// a value are in the same wavefront.  The LDS executes the operations of one
+// wavefront in the order they are issued, so a read issued after a write sees
+// it: ordering the two at wavefront scope is enough, which costs no wait
+// (a workgroup-scope barrier would wait for every outstanding LDS write).
+#ifndef POS_WSYNC_WORKGROUP
+#define POS_WSYNC_WORKGROUP 1
+#endif
+__device__ __forceinline__ void wsync() {
+#if POS_WSYNC_WORKGROUP
+    __syncthreads();
+#else
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#endif
+}
 
 // The lane group of `lane` in wave `wave`: its device system, first lane and
 // this lane's index in it.  A lane past the wave's groups gets gvalid =
@@ -215,7 +232,14 @@ enum : uint32_t {
     F_DISK = 1u << 10,        // the system has the disk effect
     F_DISK_ROLE = 1u << 11,   // this body is a disk orbiting body
     F_DYN_HOST = 1u << 12,    // the host has an evolving-dissipation model
-    F_DYN = 1u << 13          // this body has an evolving-dissipation model
+    F_DYN = 1u << 13,         // this body has an evolving-dissipation model
+    // an orbiting body of the effect (TidesEffect / RotationalFlatteningEffect /
+    // GeneralRelativityEffect::OrbitingBody) in a system where it is on; the
+    // effect loops of the reference skip every other body
+    // (constant_time_lag.rs:217-218, oblate_spheroid.rs:24, general_relativity.rs:96-97)
+    F_TIDES_ROLE = 1u << 14,
+    F_RF_ROLE = 1u << 15,
+    F_GR_ROLE = 1u << 16
 };
 struct Lane {
     int lane, gbase, k, n, host, coord, sys, body;
@@ -223,6 +247,7 @@ struct Lane {
     int tw, evo, left_index, hcount;
     long long iteration;
     double t, dt, last_hist;
+    double hist_period, time_limit;   // per-system constants, read once per launch
     double m, inv_m;
     double x[3], v[3], L[3], ev[3], eL[3];
     // alternative coordinates of this body; com* hold the host slot (centre of mass)
@@ -236,6 +261,9 @@ struct Lane {
     __device__ __forceinline__ bool tides() const { return flags & F_TIDES; }
     __device__ __forceinline__ bool rf() const { return flags & F_RF; }
     __device__ __forceinline__ bool gr() const { return flags & F_GR; }
+    __device__ __forceinline__ bool tides_role() const { return flags & F_TIDES_ROLE; }
+    __device__ __forceinline__ bool rf_role() const { return flags & F_RF_ROLE; }
+    __device__ __forceinline__ bool gr_role() const { return flags & F_GR_ROLE; }
     __device__ __forceinline__ bool spin_on() const { return flags & F_SPIN; }
     __device__ __forceinline__ int host_body() const { return body - k + host; }
     __device__ __forceinline__ int hl() const { return gbase + host; }
@@ -897,7 +925,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         //      pair_ph, and a body that is not excited removes the host's
         //      entry, as the reference does (157-160). ----
         double sigma_host = lds.sigma[hl], sigma_pair = lds.sigma[lane];
-        if (EXT && B.tides() && (B.flags & (F_DYN_HOST | F_DYN))) {
+        if (EXT && B.tides_role() && (B.flags & (F_DYN_HOST | F_DYN))) {
             const double gm = mg_of(lds, hl) + mg_of(lds, lane);
             double q, e;
             perihelion_distance_and_eccentricity(gm, px, pv, q, e);
@@ -926,20 +954,24 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         if (H) {
             os = geo.os;
             op = geo.op;
-        } else if (B.tides()) {
+        } else if (B.tides_role()) {
             os = divr(4.5 * p2(m) * p10(hR) * sigma_host, Rd7);
             op = divr(4.5 * p2(hm) * p10(R) * sigma_pair, Rd7);
         }
         // ---- flattening: orthogonal components (oblate_spheroid.rs:21-49) ----
         double fs = 0., fp = 0., ofs = 0., ofp = 0.;
-        if (B.rf()) {
+        if (B.rf_role()) {
             constexpr Rcp R6{6., 1. / 6.};
             fs = divr(m * lds.k2f[hl] * hs2 * p5(hR), R6);
             ofs = -6. * fs * rs_star / (hs2 * d5);
             fp = divr(hm * lds.k2f[lane] * s2 * p5(R), R6);
             ofp = -6. * fp * rs_planet / (s2 * d5);
         }
-        if (B.tides()) {
+        // A body the tidal loops skip adds nothing to the host's reaction sums:
+        // its rows hold +0, which leaves an ordered sum that starts at +0
+        // unchanged (such a sum is never -0).
+        if (B.tides() && !B.tides_role()) put3(lds, C_TF, lane, 0., 0., 0.);
+        if (B.tides_role()) {
             // radial component (constant_time_lag.rs:266-307)
             const double hm2 = hm * hm;
             const double pm2 = m * m;
@@ -969,7 +1001,8 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
 #pragma unroll
             for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + dk[q];
         }
-        if (B.rf()) {
+        if (B.rf() && !B.rf_role()) put3(lds, C_RF, lane, 0., 0., 0.);
+        if (B.rf_role()) {
             // radial component (oblate_spheroid.rs:51-60) and force (83-97)
             const double radial = (H ? geo.m3d5 : -3. / d5) * (fp + fs)
                 + (H ? geo.c15 : divr(15., Rd7)) * (fs * rs_star * rs_star / hs2 + fp * rs_planet * rs_planet / s2);
@@ -985,7 +1018,17 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
             // torques (constant_time_lag.rs:171-204, tides/common.rs:223-261;
             // oblate_spheroid.rs:62-81, rotational_flattening/common.rs:165-201)
             double tdL[3] = {0., 0., 0.}, rdL[3] = {0., 0., 0.};
-            if (B.tides()) {
+            if (B.tides() && !B.tides_role()) {
+                // The host's tidal torque sums over every other body, whatever
+                // its role (tides/common.rs:244-255, the test is on the host's
+                // effect), and a body the tidal loops skip keeps the zero
+                // distance, position and components it was created with, so
+                // its term is 0 * (0 * s - 0 * 0 / 0 - 1 / 0 * 0) = NaN
+                // (constant_time_lag.rs:171-204).
+                const double nan = __builtin_nan("");
+                put3(lds, C_TN, lane, nan, nan, nan);
+            }
+            if (B.tides_role()) {
                 const double cx = px[1] * pv[2] - px[2] * pv[1];
                 const double cy = px[2] * pv[0] - px[0] * pv[2];
                 const double cz = px[0] * pv[1] - px[1] * pv[0];
@@ -998,7 +1041,8 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
                      -1.0 * (os * (d * hs[1] - divr(rs_star * px[1], Rd) - id * cy)),
                      -1.0 * (os * (d * hs[2] - divr(rs_star * px[2], Rd) - id * cz)));
             }
-            if (B.rf()) {
+            if (B.rf() && !B.rf_role()) put3(lds, C_RN, lane, 0., 0., 0.);   // its torque term is -1 * 0 * 0
+            if (B.rf_role()) {
                 rdL[0] = -1.0 * (ofp * (px[1] * s[2] - px[2] * s[1]));
                 rdL[1] = -1.0 * (ofp * (px[2] * s[0] - px[0] * s[2]));
                 rdL[2] = -1.0 * (ofp * (px[0] * s[1] - px[1] * s[0]));
@@ -1010,7 +1054,13 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
 #pragma unroll
             for (int q = 0; q < 3; ++q) dLdt[q] = tdL[q] + rdL[q];
         }
-        if (GRV == GRV_KIDDER && B.gr()) {
+        if (GRV == GRV_KIDDER && B.gr() && !B.gr_role()) {   // skipped by the Kidder loops
+            put3(lds, C_G1, lane, 0., 0., 0.);
+            put3(lds, C_G2, lane, 0., 0., 0.);
+            put3(lds, C_G3, lane, 0., 0., 0.);
+            put3(lds, C_GL, lane, 0., 0., 0.);
+        }
+        if (GRV == GRV_KIDDER && B.gr_role()) {
             // Kidder 1995 (general_relativity.rs:177-456)
             const Rcp Rhm{hm, lds.inv_m[hl]};
             const Rcp Rm{m, B.inv_m};
@@ -1774,8 +1824,8 @@ __device__ inline void write_history_record(const DevEnsemble &E, WaveLds &lds, 
         lds.xb[3][lane] = B.v[0]; lds.xb[4][lane] = B.v[1]; lds.xb[5][lane] = B.v[2];
     }
     wsync();
-    double dedt = 0.;
-    if (B.orbiting() && B.tides()) {
+    double dedt = 0.;   // a body the tidal loops skip keeps its initial 0 (tides/common.rs:263-281)
+    if (B.tides_role()) {
         const int hl = B.hl();
         double r[3], u[3];
 #pragma unroll
@@ -1859,7 +1909,7 @@ template <bool REC, int V>
 __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, const TapeRef &T, int row) {
     {
         const bool first = B.last_hist < 0.;
-        const double period = E.hist_period[B.sys];
+        const double period = B.hist_period;
         if (first || B.last_hist + period <= B.t) {
             if (B.flags & F_EVOLUTION) evolve_body(E, lds, B, B.t);
             own_spin(lds, B);
@@ -1900,7 +1950,7 @@ __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, cons
     B.t = B.t + B.dt;
     B.iteration += 1;
     merge_status(lds, B);
-    if (B.t + B.dt > E.time_limit[B.sys]) B.status |= POS_STATUS_COMPLETED;
+    if (B.t + B.dt > B.time_limit) B.status |= POS_STATUS_COMPLETED;
 }
 
 }  // namespace posdev

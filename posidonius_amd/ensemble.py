@@ -82,6 +82,12 @@ class Ensemble:
         abi.check(self._lib.pos_get_options(self._h, C.byref(o)))
         return o
 
+    def launch_stats(self) -> Dict[str, float]:
+        """What the handle has executed so far (pos_get_launch_stats)."""
+        st = abi.LaunchStats()
+        abi.check(self._lib.pos_get_launch_stats(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in abi.LaunchStats._fields_}
+
     def set_time_limit(self, time_limit: float):
         abi.check(self._lib.pos_set_time_limit(self._h, float(time_limit)))
 

@@ -25,6 +25,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -127,6 +128,7 @@ struct Case {
     std::vector<double> tab_time, tab_radius, tab_rg2, tab_love, tab_itq;
     std::vector<double> disk_properties = std::vector<double>(6, 0.);   // the disk host's DiskProperties
     std::vector<double> pair_factor;   // per body: its pair-dependent map entry with the host, NaN = none
+    std::vector<double> roche;         // per body: POS_MAX_BODIES Roche radii (universe.roche_radiuses)
     int host = 0;                      // most massive body
 
     explicit Case(Value d) : doc(std::move(d)) {
@@ -239,6 +241,12 @@ struct Case {
             }
         }
         read_pair_map(u);
+        // universe.roche_radiuses: row i of an n x n block (universe.rs:177-196)
+        roche.assign((size_t)n * POS_MAX_BODIES, 0.);
+        if (const Value *rr = u.find("roche_radiuses"))
+            if (rr->is_array() && rr->items.size() >= (size_t)n * n)
+                for (int i = 0; i < n; ++i)
+                    for (int j = 0; j < n; ++j) roche[(size_t)i * POS_MAX_BODIES + j] = (*rr)[(size_t)i * n + j].num();
     }
 
     // the map entries id * MAX_PARTICLES + host_id (constant_time_lag.rs:188-194)
@@ -299,6 +307,7 @@ struct Case {
         d.evolver_inverse_tidal_q_factor = tab_itq.data();
         d.disk_properties = disk_properties.data();
         d.pair_dissipation_factor = pair_factor.data();
+        d.roche_radius = roche.data();
         return d;
     }
 };
@@ -314,13 +323,14 @@ void write_recovery_snapshot(const std::string &path, Case &c, pos_ens *ens, dou
     std::vector<double> x(3 * n), v(3 * n), L(3 * n), s(3 * n), ev(3 * n), eL(3 * n), R(n), rg2(n), I(n), t(1), lh(1);
     std::vector<int64_t> it(1);
     std::vector<int32_t> left(n), tw(1);
-    std::vector<double> pairs(n);
+    std::vector<double> pairs(n), roche((size_t)n * POS_MAX_BODIES);
     pos_state st{};
     st.position = x.data(); st.velocity = v.data(); st.angular_momentum = L.data(); st.spin = s.data();
     st.velocity_error = ev.data(); st.angular_momentum_error = eL.data(); st.radius = R.data();
     st.radius_of_gyration_2 = rg2.data(); st.moment_of_inertia = I.data(); st.current_time = t.data();
     st.last_historic_snapshot_time = lh.data(); st.current_iteration = it.data();
     st.evolver_left_index = left.data(); st.timestep_warning = tw.data(); st.pair_dissipation_factor = pairs.data();
+    st.roche_radius = roche.data();
     check(pos_get_state(ens, &st), "pos_get_state");
     Value &d = c.doc;
     put_member(d, "current_time", Value::number(t[0]));
@@ -361,6 +371,17 @@ void write_recovery_snapshot(const std::string &path, Case &c, pos_ens *ens, dou
             map.members.emplace_back(std::to_string((long long)c.ids[i] * MAX_PARTICLES + c.ids[c.host]),
                                      Value::number(pairs[i]));
     put_member(u, "pair_dependent_scaled_dissipation_factor", std::move(map));
+    {   // the Roche radii computed at t = 0, row i of an n x n block in a
+        // MAX_PARTICLES^2 array (universe.rs:62, 177-196)
+        Value rr;
+        rr.kind = Value::Array;
+        const size_t len = std::max<size_t>((size_t)(MAX_PARTICLES * MAX_PARTICLES), (size_t)n * n);
+        for (size_t k = 0; k < len; ++k) {
+            const size_t i = k / n, j = k % n;
+            rr.items.push_back(Value::number(k < (size_t)n * n ? roche[i * POS_MAX_BODIES + j] : 0.));
+        }
+        put_member(u, "roche_radiuses", std::move(rr));
+    }
     if (exists(path)) std::rename(path.c_str(), (path + ".previous").c_str());
     std::string out;
     posjson::dump(d, out, 2);

@@ -72,11 +72,54 @@ def test_invalid_desc_is_rejected_before_any_device_work(lib):
     rc = lib.pos_create(h.ref, None, C.byref(out))
     assert rc == abi.E_INVALID
     assert b"disk_properties" in lib.pos_last_error()
-    # properties given, but the host is no disk central body
+    # properties given, but the disk's central body is a planet
     spec.system["disk_properties"] = np.tile([0.01, 100.0, 3.6525e7, 0.01, 1e-4, 2.4], (spec.n_systems, 1))
+    planet = spec.body_offset[2] + 1
+    spec.body["roles"][planet] |= abi.ROLE_DISK_CENTRAL
     h = spec.desc()
     assert lib.pos_create(h.ref, None, C.byref(out)) == abi.E_UNSUPPORTED
     assert b"disk host" in lib.pos_last_error()
+
+
+def test_central_body_rules_of_the_reference(lib):
+    """find_indices / disable_unnecessary_effects (universe.rs:699-1006): one
+    central body per effect, the GR host is the most massive body, tides and
+    flattening share their host; the reference panics on each of these."""
+    from posidonius_amd import generators as gen
+    out = C.c_void_p()
+    for role, text in ((abi.ROLE_TIDES_CENTRAL, b"only one central body"),
+                       (abi.ROLE_GR_CENTRAL, b"only one central body")):
+        spec = gen.config1(4)
+        spec.body["roles"][spec.body_offset[1] + 2] |= role
+        assert lib.pos_create(spec.desc().ref, None, C.byref(out)) == abi.E_INVALID
+        assert text in lib.pos_last_error()
+    spec = gen.config1(4)   # GR central body moved to a planet
+    b0 = spec.body_offset[3]
+    spec.body["roles"][b0] &= ~np.uint32(abi.ROLE_GR_CENTRAL)
+    spec.body["roles"][b0 + 1] = (spec.body["roles"][b0 + 1] & ~np.uint32(abi.ROLE_GR_ORBITING)) | abi.ROLE_GR_CENTRAL
+    assert lib.pos_create(spec.desc().ref, None, C.byref(out)) == abi.E_INVALID
+    assert b"most massive" in lib.pos_last_error()
+    spec = gen.config1(4)   # flattening central body on a planet, tides on the star
+    b0 = spec.body_offset[0]
+    spec.body["roles"][b0] &= ~np.uint32(abi.ROLE_FLAT_CENTRAL)
+    spec.body["roles"][b0 + 1] = (spec.body["roles"][b0 + 1] & ~np.uint32(abi.ROLE_FLAT_ORBITING)) | abi.ROLE_FLAT_CENTRAL
+    assert lib.pos_create(spec.desc().ref, None, C.byref(out)) == abi.E_INVALID
+    assert b"tidal & rotational flattening" in lib.pos_last_error()
+
+
+def test_disabled_orbiting_roles_pass_validation(lib):
+    """A body whose tides, flattening or GR is Disabled (the Python builder's
+    per-body default, posidonius/particles/particle.py:52-56) is accepted:
+    without a GPU the call gets past validation to the device check."""
+    from posidonius_amd import generators as gen
+    spec = gen.config1(4)
+    for s, role in enumerate((abi.ROLE_TIDES_ORBITING, abi.ROLE_FLAT_ORBITING, abi.ROLE_GR_ORBITING)):
+        spec.body["roles"][spec.body_offset[s] + 1] &= ~np.uint32(role)
+    out = C.c_void_p()
+    rc = lib.pos_create(spec.desc().ref, None, C.byref(out))
+    assert rc in (abi.E_NO_DEVICE, abi.OK), lib.pos_last_error()
+    if rc == abi.OK:
+        lib.pos_destroy(out)
 
 
 def test_evolving_dissipation_needs_its_q_factor_table(lib):

@@ -7,6 +7,8 @@ J d is the tangent-linear (variational) solution of the restated step (the
 oracle instantiated with dual numbers).  Directions and seeds are taken one
 block at a time so that an error in any single block shows on its own.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -51,12 +53,16 @@ def per_system(spec, a):
     return np.add.reduceat(a, spec.body_offset[:-1])
 
 
-@pytest.mark.parametrize("steps,segment", [(200, 64), (37, 1000)])
+@pytest.mark.parametrize("steps,segment", [(200, 64), (37, 1000), (10_000, 1000), (100_000, 1000)])
 def test_adjoint_equals_tangent_linear(Ensemble, oracle, steps, segment):
+    """16 systems of the adjoint workload, up to its full horizon (10^5
+    steps with a checkpoint every 1,000): every seed block against every
+    tangent direction to 1e-6 relative."""
     spec = gen.config4(16)
+    print(f"adjoint vs tangent-linear: {spec.n_systems} systems x {steps} steps, segments of {segment}", flush=True)
     rng = np.random.default_rng(17)
     d = directions(spec, rng)
-    ref, tan = oracle.tangent(spec, steps, d)
+    ref, tan = oracle.tangent(spec, steps, d, threads=max(1, min(16, os.cpu_count() or 1)))
     with Ensemble(spec, mapping="packed", steps_per_launch=64) as ens:
         ens.adjoint_record(steps, segment_steps=segment)
         fin = ens.get_state()

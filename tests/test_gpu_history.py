@@ -85,3 +85,52 @@ def test_history_with_graph_replay_equals_direct_launches():
         np.testing.assert_array_equal(rec1[name], rec0[name])
     for k in st0:
         np.testing.assert_array_equal(st1[k], st0[k])
+
+
+def _records_equal(a, b):
+    assert len(a) == len(b) > 0
+    for name in a.dtype.names:
+        np.testing.assert_array_equal(a[name], b[name], err_msg=name)
+
+
+def test_tuning_appends_no_history_records():
+    """pos_tune probes the launch mappings on a copy of the state; the probe
+    steps are not part of the run, so the history holds only the caller's
+    steps and equals that of an untuned handle."""
+    from posidonius_amd.ensemble import Ensemble
+    spec = gen.config2(256, seed=5)
+    spec.system["historic_snapshot_period"][:] = 3.5 * spec.system["time_step"]
+    out = {}
+    for tuned in (False, True):
+        with Ensemble(spec, mapping="packed", group_size=-1, steps_per_launch=16, use_graph=True) as ens:
+            ens.enable_history(64)
+            ens.step(10)
+            if tuned:
+                ens.tune(probe_steps=16)
+            ens.step(70)
+            out[tuned] = (ens.drain_history(), ens.get_state(fields=("position", "status")))
+    _records_equal(out[True][0], out[False][0])
+    np.testing.assert_array_equal(out[True][1]["position"], out[False][1]["position"])
+    assert not np.any(out[True][1]["status"] & abi.STATUS_HISTORY_OVERFLOW)
+
+
+def test_adjoint_appends_no_history_records():
+    """pos_adjoint recomputes every recorded segment from its checkpoint;
+    those steps were recorded once already and add no history records."""
+    from posidonius_amd.ensemble import Ensemble
+    spec = gen.config4(32)
+    spec.system["historic_snapshot_period"][:] = 9.5 * spec.system["time_step"]
+    rng = np.random.default_rng(3)
+    w = rng.standard_normal((spec.n_bodies, 3))
+    with Ensemble(spec, mapping="packed", steps_per_launch=16) as ens:
+        ens.enable_history(64)
+        ens.step(200)
+        plain = ens.drain_history()
+    with Ensemble(spec, mapping="packed", steps_per_launch=16) as ens:
+        ens.enable_history(64)
+        ens.adjoint_record(200, segment_steps=50)
+        ens.adjoint(position=w)
+        adj = ens.drain_history()
+        after = ens.get_state(fields=("status",))
+    _records_equal(adj, plain)
+    assert not np.any(after["status"] & abi.STATUS_HISTORY_OVERFLOW)

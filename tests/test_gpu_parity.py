@@ -17,6 +17,7 @@ from tests.oracle_runner import golden_names, load_golden
 pytestmark = pytest.mark.gpu
 
 SUPPORTED = golden_names()
+THREADS = max(1, min(16, os.cpu_count() or 1))   # host threads for the oracle
 STATE_KEYS = ("position", "velocity", "acceleration", "angular_momentum", "spin", "velocity_error",
               "angular_momentum_error", "radius", "moment_of_inertia", "current_time", "current_iteration",
               "status", "evolver_left_index", "timestep_warning", "pair_dissipation_factor")
@@ -78,11 +79,13 @@ def test_fixtures_against_oracle_full_state(Ensemble, oracle, mapping):
     assert_same_state(out, ref, where=f"[{mapping}]")
 
 
-@pytest.mark.parametrize("cfg,n,steps", [(0, None, 1000), (1, 256, 1000), (2, 512, 300), (3, 128, 300), (4, 64, 1000)])
+@pytest.mark.parametrize("cfg,n,steps", [(0, None, 1000), (1, 256, 1000), (2, 1024, 1000), (3, 512, 1000), (4, 64, 1000)])
 def test_configs_bitwise_against_oracle(Ensemble, oracle, cfg, n, steps):
+    """Every benchmark configuration at the 10^3-step horizon of the parity
+    target (1e-10 relative); the device meets it bit for bit."""
     spec = gen.CONFIGS[cfg]() if n is None else gen.CONFIGS[cfg](n)
     log(f"config {cfg}: {spec.n_systems} systems x {steps} steps")
-    ref = oracle.run(spec, steps, threads=8)
+    ref = oracle.run(spec, steps, threads=THREADS)
     with Ensemble(spec, mapping="packed", steps_per_launch=64) as ens:
         ens.step(steps)
         out = ens.get_state()
@@ -99,7 +102,7 @@ def test_launch_mappings_agree(Ensemble, oracle, group_size, K, graph):
     if group_size == -2:   # sizes 2-16 mixed in wavefronts
         spec = gen.config2(256, seed=43, n_min=2, n_max=abi.MAX_BODIES)
     steps = 100
-    ref = oracle.run(spec, steps, threads=8)
+    ref = oracle.run(spec, steps, threads=THREADS)
     log(f"mapping G={group_size} K={K} graph={graph}")
     with Ensemble(spec, mapping="packed", group_size=group_size, steps_per_launch=K, use_graph=graph) as ens:
         ens.step(steps)
@@ -168,26 +171,26 @@ def test_collision_freezes_system(Ensemble, oracle):
     assert_same_state(out, ref, where="collision")
 
 
-def test_large_ensemble_subset_properties(Ensemble, oracle):
-    """At the benchmark size (65,536 systems of 2-12 bodies) the oracle is too
-    slow for every system; check that (a) a random subset equals the oracle
-    bit for bit and (b) the run is deterministic across launch mappings."""
+def test_benchmark_ensemble_at_horizon(Ensemble, oracle):
+    """The benchmark ensemble itself (65,536 systems of 2-12 bodies) for 10^3
+    steps: every system equals the oracle bit for bit under the benchmark's
+    launch shape (exactly N lanes per system, 64 steps per launch, graph
+    replay), and a second mapping (16 lanes per system, 16 steps per launch,
+    direct launches) gives the same bits."""
     spec = gen.config2(65536)
-    steps = 64
-    with Ensemble(spec, steps_per_launch=32) as ens:
+    steps = 1000
+    log(f"benchmark ensemble: {spec.n_systems} systems x {steps} steps")
+    with Ensemble(spec, group_size=-1, steps_per_launch=64, use_graph=True) as ens:
         ens.step(steps)
         out = ens.get_state()
-    with Ensemble(spec, steps_per_launch=16, group_size=16) as ens:
+    with Ensemble(spec, group_size=16, steps_per_launch=16) as ens:
         ens.step(steps)
         out2 = ens.get_state()
     assert_same_state(out2, out, where="determinism")
-    rng = np.random.default_rng(7)
-    pick = np.sort(rng.choice(spec.n_systems, 64, replace=False))
-    sub = spec.subset(pick)
-    ref = oracle.run(sub, steps, threads=8)
-    idx = np.concatenate([np.arange(spec.body_offset[s], spec.body_offset[s + 1]) for s in pick])
-    sub_out = {k: (v[pick] if v.shape[0] == spec.n_systems else v[idx]) for k, v in out.items()}
-    assert_same_state(sub_out, ref, where="subset")
+    log(f"oracle on {THREADS} threads")
+    ref = oracle.run(spec, steps, threads=THREADS)
+    assert_same_state(out, ref, where="benchmark ensemble")
+    log(f"{int(np.count_nonzero(out['current_iteration'] < steps))} systems stopped early (fatal status)")
 
 
 def gr_variant_ensemble():
@@ -211,7 +214,7 @@ def test_gr_variants_against_oracle(Ensemble, oracle, mapping, group_size, graph
     with the oracle bit for bit."""
     spec = gr_variant_ensemble()
     steps = 80
-    ref = oracle.run(spec, steps, threads=8)
+    ref = oracle.run(spec, steps, threads=THREADS)
     log(f"GR variants mapping={mapping} G={group_size} graph={graph}: {spec.n_systems} systems x {steps} steps")
     with Ensemble(spec, mapping=mapping, group_size=group_size, steps_per_launch=16, use_graph=graph) as ens:
         ens.step(steps)
@@ -287,7 +290,7 @@ def test_extended_effects_against_oracle(Ensemble, oracle, mapping, group_size, 
     kernel variant: bitwise equal to the oracle."""
     spec = extended_ensemble()
     steps = 150
-    ref = oracle.run(spec, steps, threads=8)
+    ref = oracle.run(spec, steps, threads=THREADS)
     log(f"extended effects mapping={mapping} G={group_size} graph={graph}: {spec.n_systems} systems x {steps} steps")
     with Ensemble(spec, mapping=mapping, group_size=group_size, steps_per_launch=16, use_graph=graph) as ens:
         ens.step(steps)
@@ -450,7 +453,7 @@ def test_fixed_effect_set_with_the_host_anywhere(Ensemble, oracle, mapping, grou
     assert np.any(spec.body["mass"][spec.body_offset[:-1]] < 0.05), "some star must have moved"
     steps = 200
     log(f"host anywhere: {spec.n_systems} systems x {steps} steps, {mapping} G={group_size}")
-    ref = oracle.run(spec, steps, threads=8)
+    ref = oracle.run(spec, steps, threads=THREADS)
     with Ensemble(spec, mapping=mapping, group_size=group_size, steps_per_launch=64) as ens:
         ens.step(steps)
         out = ens.get_state()
@@ -465,7 +468,7 @@ def test_tune_restores_the_state_and_keeps_the_bits(Ensemble, oracle, n_max):
     packing is among the trials."""
     spec = gen.config2(192, seed=7, n_min=2, n_max=n_max)
     steps = 60
-    ref = oracle.run(spec, steps, threads=8)
+    ref = oracle.run(spec, steps, threads=THREADS)
     log(f"tune: {spec.n_systems} systems of 2-{n_max} bodies")
     with Ensemble(spec, mapping="packed", use_graph=True) as ens:
         ens.step(20)
@@ -477,3 +480,41 @@ def test_tune_restores_the_state_and_keeps_the_bits(Ensemble, oracle, n_max):
         ens.step(steps - 20)
         out = ens.get_state()
     assert_same_state(out, ref, where=f"tuned G={o.group_size} K={o.steps_per_launch}")
+
+
+def test_resume_keeps_the_roche_radii_of_t0(Ensemble, oracle):
+    """Roche radii are computed once, at t = 0 (initialize_physical_values,
+    whfast.rs:226-233), and a resumed run keeps them (they travel in the
+    snapshot, universe.rs:62).  Planet 2 of the fixture has an evolving
+    radius, so radii recomputed at the resume would differ from them."""
+    spec, _ = load_golden("test_evolution-brown_dwarf_leconte2011")
+    spec.system["time_limit"][:] = 1e9
+    with Ensemble(spec) as ens:
+        r0 = ens.get_state(fields=("roche_radius", "radius"))
+        ens.step(150)
+        mid = ens.get_state()
+        ens.step(150)
+        full = ens.get_state()
+    np.testing.assert_array_equal(mid["roche_radius"], r0["roche_radius"])
+    assert not np.array_equal(mid["radius"], r0["radius"]), "the fixture's radii must evolve"
+    resumed = spec.copy()
+    for k in ("position", "velocity", "angular_momentum", "spin", "velocity_error", "angular_momentum_error",
+              "radius", "radius_of_gyration_2", "moment_of_inertia", "evolver_left_index", "roche_radius"):
+        resumed.body[k] = mid[k].copy()
+    for k in ("current_time", "last_historic_snapshot_time", "current_iteration"):
+        resumed.system[k] = mid[k].copy()
+    with Ensemble(resumed) as ens2:
+        np.testing.assert_array_equal(ens2.get_state(fields=("roche_radius",))["roche_radius"], r0["roche_radius"])
+        ens2.set_state({"acceleration": mid["acceleration"]})
+        ens2.step(150)
+        out = ens2.get_state()
+    assert_same_state(out, full, keys=("position", "velocity", "angular_momentum", "spin", "radius",
+                                       "roche_radius", "current_time", "current_iteration"))
+    # the oracle keeps the carried radii too
+    ref = oracle.run(resumed, 150)
+    np.testing.assert_array_equal(ref["roche_radius"], r0["roche_radius"])
+    # without them, a resumed system gets radii from its current radii
+    resumed.body["roche_radius"] = None
+    with Ensemble(resumed) as ens3:
+        r3 = ens3.get_state(fields=("roche_radius",))["roche_radius"]
+    assert not np.array_equal(r3, r0["roche_radius"])
