@@ -89,6 +89,11 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
     B.dt = E.dt[s];
     B.hist_period = E.hist_period[s];
     B.time_limit = E.time_limit[s];
+    B.roche_max2 = -1.;   // no pair: no distance is below it
+    for (int j = k + 1; valid && j < B.n; ++j) {
+        const double rr = E.roche[(size_t)b0 * POS_MAX_BODIES + (size_t)k * POS_MAX_BODIES + j];
+        B.roche_max2 = fmax(B.roche_max2, rr * rr);
+    }
     B.t = E.t[s];
     B.last_hist = E.last_hist[s];
     B.iteration = E.iteration[s];
@@ -872,6 +877,28 @@ int ensure_graph(pos_ens *h) {
     return POS_OK;
 }
 
+// One launch of effects_kernel per kernel variant present, writing device-order
+// results to d_a, d_l ([3][n_bodies] each).
+int launch_effects(pos_ens *h, double *d_a, double *d_l) {
+    for (int v = 0; v < N_VARIANTS; ++v) {
+        const int w0 = h->variant_waves[v][0], w1 = h->variant_waves[v][1];
+        if (w1 <= w0) continue;
+        const dim3 g(w1 - w0);
+        switch (v) {
+            case 0: hipLaunchKernelGGL(effects_kernel<0>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 1: hipLaunchKernelGGL(effects_kernel<1>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 2: hipLaunchKernelGGL(effects_kernel<2>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 4: hipLaunchKernelGGL(effects_kernel<4>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 5: hipLaunchKernelGGL(effects_kernel<5>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 6: hipLaunchKernelGGL(effects_kernel<6>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 8: hipLaunchKernelGGL(effects_kernel<8>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            default: return fail(POS_E_INVALID, "internal error: unknown kernel variant");
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    return POS_OK;
+}
+
 // Device time between the two operator events (both recorded and completed).
 int record_operator_time(pos_ens *h) {
     float ms = 0.f;
@@ -1361,22 +1388,7 @@ int32_t pos_additional_effects(pos_ens *h, double *out_acc, double *out_dLdt) {
         return rc;
     double *d_a = h->d_op_a, *d_l = h->d_op_l;
     HIP_TRY(hipEventRecord(h->op_ev[0], h->stream));
-    for (int v = 0; v < N_VARIANTS; ++v) {
-        const int w0 = h->variant_waves[v][0], w1 = h->variant_waves[v][1];
-        if (w1 <= w0) continue;
-        const dim3 g(w1 - w0);
-        switch (v) {
-            case 0: hipLaunchKernelGGL(effects_kernel<0>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
-            case 1: hipLaunchKernelGGL(effects_kernel<1>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
-            case 2: hipLaunchKernelGGL(effects_kernel<2>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
-            case 4: hipLaunchKernelGGL(effects_kernel<4>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
-            case 5: hipLaunchKernelGGL(effects_kernel<5>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
-            case 6: hipLaunchKernelGGL(effects_kernel<6>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
-            case 8: hipLaunchKernelGGL(effects_kernel<8>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
-            default: return fail(POS_E_INVALID, "internal error: unknown kernel variant");
-        }
-        HIP_TRY(hipGetLastError());
-    }
+    if ((rc = launch_effects(h, d_a, d_l))) return rc;
     HIP_TRY(hipEventRecord(h->op_ev[1], h->stream));
     if ((rc = download_bodies(h, out_acc, d_a, 3)) || (rc = download_bodies(h, out_dLdt, d_l, 3))) return rc;
     return record_operator_time(h);

@@ -169,7 +169,7 @@ struct WaveLds {
 // it: ordering the two at wavefront scope is enough, which costs no wait
 // (a workgroup-scope barrier would wait for every outstanding LDS write).
 #ifndef POS_WSYNC_WORKGROUP
-#define POS_WSYNC_WORKGROUP 1
+#define POS_WSYNC_WORKGROUP 0
 #endif
 __device__ __forceinline__ void wsync() {
 #if POS_WSYNC_WORKGROUP
@@ -248,6 +248,10 @@ struct Lane {
     long long iteration;
     double t, dt, last_hist;
     double hist_period, time_limit;   // per-system constants, read once per launch
+    // the largest squared Roche radius of this body's pairs with the bodies
+    // after it (rr * rr, as the Roche test squares it), read once per launch:
+    // a pair farther apart than that cannot meet its own Roche test
+    double roche_max2;
     double m, inv_m;
     double x[3], v[3], L[3], ev[3], eL[3];
     // alternative coordinates of this body; com* hold the host slot (centre of mass)
@@ -1772,8 +1776,11 @@ __device__ inline void gravity(const DevEnsemble &E, WaveLds &lds, Lane &B) {
             const double dz = B.x[2] - lds.xb[2][l];
             const double distance_2 = dx * dx + dy * dy + dz * dz;
             if (i < j) {
-                const double rr = E.roche[(size_t)B.body * POS_MAX_BODIES + j];
-                if (distance_2 <= rr * rr) B.status |= POS_STATUS_ROCHE;
+                // the table is read only when the screen cannot exclude the pair
+                if (distance_2 <= B.roche_max2) {
+                    const double rr = E.roche[(size_t)B.body * POS_MAX_BODIES + j];
+                    if (distance_2 <= rr * rr) B.status |= POS_STATUS_ROCHE;
+                }
                 if (distance_2 <= p2(Ri + lds.R[l])) B.status |= POS_STATUS_COLLISION;
                 if (i == mm && distance_2 > MAX_DISTANCE_2) B.status |= POS_STATUS_EJECTION;
             }
