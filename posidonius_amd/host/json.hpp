@@ -6,6 +6,7 @@
 // integers, e.g. the snapshot hash, that a double would not round-trip).
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -58,6 +59,7 @@ struct Value {
         return items[i];
     }
     double num() const {
+        if (kind == Null) return std::nan("");   // serde_json writes a non-finite f64 as null
         if (kind != Number) throw std::runtime_error("expected a number");
         return std::strtod(text.c_str(), nullptr);
     }
@@ -72,6 +74,7 @@ struct Value {
 
     static Value number(double x) {
         Value v;
+        if (!std::isfinite(x)) return v;   // null, as serde_json writes NaN and infinities
         v.kind = Number;
         char buf[40];
         std::snprintf(buf, sizeof buf, "%.17g", x);

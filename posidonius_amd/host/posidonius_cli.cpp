@@ -5,13 +5,17 @@
 //   posidonius-hip resume <snapshot.json> <history.bin> [-l seconds] [-s]
 //                         [--historic-snapshot-period days] [--recovery-snapshot-period days]
 //                         [--time-limit days]
+//   posidonius-hip convert <in> <out>       (JSON <-> bincode, by extension)
 //   common options: [--device N] [--history-capacity N]
 //
 // Files:
-//   * case / recovery snapshot: the WHFast JSON document the reference's
-//     Python package writes and its serde_json snapshots use
-//     (src/integrator/output.rs:28-54, 178-227); a snapshot is the input
-//     document with the integrator and particle state updated;
+//   * case / recovery snapshot: the WHFast document the reference's Python
+//     package writes and its snapshots hold (src/integrator/output.rs:28-54,
+//     178-279), as JSON when the file name ends in ".json" and as bincode
+//     1.x otherwise (bincode.hpp); a snapshot is the input document with the
+//     integrator and particle state updated.  Before a snapshot is
+//     replaced, the old one is kept as <stem>.<YYYYMMDD>T<AM|PM>.bin, one per
+//     twelve hours (output.rs:35-41);
 //   * history: the reference's binary records, 156 bytes per body and
 //     snapshot (output.rs:57-172, bincode little-endian):
 //       f64 time, f64 time_step, i32 id, f64 x[3], spin[3], v[3], mass,
@@ -40,6 +44,7 @@
 #include <vector>
 
 #include "../../include/posidonius_hip.h"
+#include "bincode.hpp"
 #include "json.hpp"
 
 using posjson::Value;
@@ -69,6 +74,37 @@ std::string read_file(const std::string &p) {
     std::stringstream ss;
     ss << f.rdbuf();
     return ss.str();
+}
+
+// The extension decides the format, as in the reference (output.rs:48, 186).
+bool is_json_path(const std::string &p) {
+    const size_t slash = p.find_last_of('/');
+    const size_t dot = p.find_last_of('.');
+    return dot != std::string::npos && (slash == std::string::npos || dot > slash) && p.substr(dot + 1) == "json";
+}
+
+Value read_document(const std::string &p) {
+    const std::string bytes = read_file(p);
+    return is_json_path(p) ? posjson::parse(bytes) : posbin::decode(bytes);
+}
+
+// Path::with_extension(ext): the file name's last extension replaced.
+std::string with_extension(const std::string &p, const std::string &ext) {
+    const size_t slash = p.find_last_of('/');
+    const size_t dot = p.find_last_of('.');
+    const bool has_ext = dot != std::string::npos && (slash == std::string::npos || dot > slash + 1);
+    return (has_ext ? p.substr(0, dot) : p) + "." + ext;
+}
+
+// The backup name of an existing snapshot: one per twelve hours, UTC
+// (output.rs:35-41, time's "[year][month][day]T[period]" format).
+std::string backup_path(const std::string &p) {
+    const std::time_t t = std::time(nullptr);
+    const std::tm *g = std::gmtime(&t);
+    char buf[32];
+    std::snprintf(buf, sizeof buf, "%04d%02d%02dT%s.bin", g->tm_year + 1900, g->tm_mon + 1, g->tm_mday,
+                  g->tm_hour < 12 ? "AM" : "PM");
+    return with_extension(p, buf);
 }
 
 // serde externally tagged enum: "Tag" or {"Tag": payload}
@@ -348,6 +384,8 @@ void write_recovery_snapshot(const std::string &path, Case &c, pos_ens *ens, dou
     Value *evs = u.find("particles_evolvers");
     for (int i = 0; i < n; ++i) {
         Value &p = ps[i];
+        // every particle carries its id, its index at creation (universe.rs:119)
+        put_member(p, "id", Value::integer_value(c.ids[i]));
         set_axes(p.at("inertial_position"), &x[3 * i]);
         set_axes(p.at("inertial_velocity"), &v[3 * i]);
         set_axes(p.at("angular_momentum"), &L[3 * i]);
@@ -382,9 +420,17 @@ void write_recovery_snapshot(const std::string &path, Case &c, pos_ens *ens, dou
         }
         put_member(u, "roche_radiuses", std::move(rr));
     }
-    if (exists(path)) std::rename(path.c_str(), (path + ".previous").c_str());
     std::string out;
-    posjson::dump(d, out, 2);
+    if (is_json_path(path)) {
+        posjson::dump(d, out, 2);
+    } else {
+        try {
+            out = posbin::encode(d);
+        } catch (const std::exception &e) {
+            die(std::string("cannot encode the bincode snapshot: ") + e.what());
+        }
+    }
+    if (exists(path) && std::rename(path.c_str(), backup_path(path).c_str()) != 0) die("cannot back up '" + path + "'");
     std::ofstream f(path, std::ios::binary);
     if (!f) die("cannot write '" + path + "'");
     f << out;
@@ -397,8 +443,30 @@ void usage() {
                  "usage: posidonius-hip start <case.json> <snapshot.json> <history.bin> [-l seconds] [-s]\n"
                  "       posidonius-hip resume <snapshot.json> <history.bin> [-l seconds] [-s]\n"
                  "              [--historic-snapshot-period days] [--recovery-snapshot-period days] [--time-limit days]\n"
-                 "       common: [--device N] [--history-capacity N]\n");
+                 "       common: [--device N] [--history-capacity N]\n"
+                 "       posidonius-hip convert <in> <out>   (a case or snapshot between JSON and bincode)\n"
+                 "  a file name ending in .json is JSON, any other is the reference's bincode layout\n");
     std::exit(2);
+}
+
+// `convert`: one document from one encoding to the other (no device work).
+int convert(const std::string &in, const std::string &out) {
+    Value doc;
+    try {
+        doc = read_document(in);
+    } catch (const std::exception &e) {
+        die(std::string("cannot read '") + in + "': " + e.what());
+    }
+    std::string bytes;
+    try {
+        if (is_json_path(out)) posjson::dump(doc, bytes, 2);
+        else bytes = posbin::encode(doc);
+    } catch (const std::exception &e) {
+        die(std::string("cannot encode '") + out + "': " + e.what());
+    }
+    std::ofstream f(out, std::ios::binary);
+    if (!f || !(f << bytes)) die("cannot write '" + out + "'");
+    return 0;
 }
 
 }  // namespace
@@ -407,6 +475,10 @@ int main(int argc, char **argv) {
     const auto wall0 = std::chrono::steady_clock::now();
     if (argc < 2) usage();
     const std::string cmd = argv[1];
+    if (cmd == "convert") {
+        if (argc != 4) usage();
+        return convert(argv[2], argv[3]);
+    }
     const bool resume = cmd == "resume";
     if (!resume && cmd != "start") usage();
     std::vector<std::string> pos;
@@ -437,7 +509,7 @@ int main(int argc, char **argv) {
 
     std::unique_ptr<Case> cp;
     try {
-        cp.reset(new Case(posjson::parse(read_file(first))));
+        cp.reset(new Case(read_document(first)));
     } catch (const std::exception &e) {
         die(std::string("cannot read '") + first + "': " + e.what());
     }
@@ -500,7 +572,11 @@ int main(int argc, char **argv) {
         int64_t k = 0;
         int snaps = 0;
         bool want_recovery = false, completes = false;
-        while (k < (1 << 20)) {
+        // with an execution-time limit the reference writes no periodic
+        // recovery snapshot (main.rs:158-176) and checks the clock every
+        // step; launches are kept short enough to check it often
+        const int64_t max_chunk = limit_s > 0 ? (1 << 14) : (1 << 20);
+        while (k < max_chunk) {
             const bool first_trigger = last_hist < 0.;
             const bool hist_trigger = first_trigger || last_hist + hist_period <= t;
             const bool rec_trigger = last_rec + rec_period <= t;
@@ -512,7 +588,7 @@ int main(int argc, char **argv) {
             t = t + dt;
             ++k;
             if (t + dt > tl) { completes = true; break; }
-            if (first_trigger || rec_trigger) { want_recovery = true; break; }
+            if (limit_s == 0 && (first_trigger || rec_trigger)) { want_recovery = true; break; }
         }
         check(pos_step(ens, k), "pos_step");
         int64_t nrec = 0;

@@ -139,3 +139,45 @@ def test_resume_with_orbiting_dynamical_tides(tmp_path):
     cli("resume", str(tmp_path / "b.json"), str(tmp_path / "b.bin"), "-s", "--time-limit", repr(limit))
     assert open(tmp_path / "b.bin", "rb").read() == open(tmp_path / "a.bin", "rb").read()
     assert len(read_history(tmp_path / "a.bin", n)) > 0
+
+
+def test_bincode_snapshots_resume_bit_for_bit(tmp_path):
+    """The reference's documented usage, `start case.json case.bin history.bin`
+    (README.md:55-58): a snapshot whose name does not end in .json is written
+    in the bincode layout (output.rs:45-52), the previous one is kept as
+    <stem>.<YYYYMMDD>T<AM|PM>.bin (35-41), and a run stopped early and resumed
+    from it writes the same history as an uninterrupted run."""
+    import glob
+    full = write_case(tmp_path / "case.json", 10, 50)
+    n = full["universe"]["n_particles"]
+    cli("start", str(tmp_path / "case.json"), str(tmp_path / "a.bin"), str(tmp_path / "a.hist"), "-s")
+    write_case(tmp_path / "short.json", 10, 50, time_limit=full["universe"]["time_limit"] * 0.6)
+    cli("start", str(tmp_path / "short.json"), str(tmp_path / "b.bin"), str(tmp_path / "b.hist"), "-s")
+    raw = open(tmp_path / "b.bin", "rb").read()
+    with pytest.raises(ValueError):     # a .bin snapshot is not JSON
+        json.loads(raw)
+    assert struct.unpack_from("<d", raw, 0)[0] == full["time_step"]
+    assert glob.glob(str(tmp_path / "b.*T[AP]M.bin")), "the previous snapshot is kept as a 12-hourly backup"
+    cli("resume", str(tmp_path / "b.bin"), str(tmp_path / "b.hist"), "-s", "--time-limit",
+        repr(full["universe"]["time_limit"]))
+    assert open(tmp_path / "b.hist", "rb").read() == open(tmp_path / "a.hist", "rb").read()
+    assert len(read_history(tmp_path / "a.hist", n)) > 0
+
+
+def test_json_snapshot_and_its_bincode_form_resume_identically(tmp_path):
+    """A JSON recovery snapshot converted to bincode (`posidonius-hip convert`)
+    resumes to the same history, byte for byte, as the JSON one."""
+    import shutil
+    full = write_case(tmp_path / "case.json", 10, 50)
+    write_case(tmp_path / "short.json", 10, 50, time_limit=full["universe"]["time_limit"] * 0.6)
+    cli("start", str(tmp_path / "short.json"), str(tmp_path / "s.json"), str(tmp_path / "s.hist"), "-s")
+    n = full["universe"]["n_particles"]
+    snap = json.load(open(tmp_path / "s.json"))
+    # the input has no ids; the snapshot holds every particle's, its index (universe.rs:119)
+    assert [p["id"] for p in snap["universe"]["particles"][:n]] == list(range(n))
+    cli("convert", str(tmp_path / "s.json"), str(tmp_path / "s.bin"))
+    shutil.copy(tmp_path / "s.hist", tmp_path / "t.hist")
+    limit = repr(full["universe"]["time_limit"])
+    cli("resume", str(tmp_path / "s.json"), str(tmp_path / "s.hist"), "-s", "--time-limit", limit)
+    cli("resume", str(tmp_path / "s.bin"), str(tmp_path / "t.hist"), "-s", "--time-limit", limit)
+    assert open(tmp_path / "t.hist", "rb").read() == open(tmp_path / "s.hist", "rb").read()
