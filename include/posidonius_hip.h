@@ -315,6 +315,21 @@ int32_t pos_tune(pos_ens *ens, int32_t probe_steps, pos_options *chosen);
 int32_t pos_gravity_acceleration(pos_ens *ens, double *out_acc, uint32_t *out_status);
 int32_t pos_additional_effects(pos_ens *ens, double *out_acc, double *out_dangular_momentum_dt);
 
+/* pos_additional_effects for callers that cannot allocate: code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc.  No
+ * counterpart in the reference.  It evaluates the same kernels, but writes
+ * into caller-owned DEVICE buffers (caller body order, [nb][3] each, on the
+ * handle's device), allocates nothing (no hipMalloc, hipFree or allocating
+ * library call; its intermediate results go to a static device scratch array
+ * of 2 x 3 x POS_NOALLOC_MAX_BODIES doubles, 24 MiB, that each GPU the library
+ * runs on holds until the process ends) and does not synchronize: the
+ * results are ready when the handle's stream (pos_get_stream) reaches this
+ * point.  Ensembles of more than POS_NOALLOC_MAX_BODIES bodies are rejected
+ * with POS_E_LIMIT.  Every call on one GPU shares that GPU's scratch array,
+ * so calls that overlap on one GPU (two handles, or two host threads) race. */
+#define POS_NOALLOC_MAX_BODIES 524288
+int32_t pos_additional_effects_noalloc(pos_ens *ens, double *dev_out_acc, double *dev_out_dangular_momentum_dt);
+
 /*
  * Historic snapshots (src/integrator/output.rs:90-172, written by
  * WHFast::iterate at whfast.rs:240-256).  After pos_history_enable, every

@@ -72,6 +72,8 @@ STATUS_ITERATION_CAP = 1 << 8
 STATUS_HISTORY_OVERFLOW = 1 << 9
 STATUS_FATAL = STATUS_ROCHE | STATUS_COLLISION | STATUS_EJECTION | STATUS_ZERO_INERTIA | STATUS_ITERATION_CAP
 
+NOALLOC_MAX_BODIES = 524288
+
 MAPPING_AUTO = 0
 MAPPING_PLAIN = 1
 MAPPING_PACKED = 2
@@ -293,6 +295,8 @@ def _declare(lib):
     lib.pos_gravity_acceleration.restype = C.c_int32
     lib.pos_additional_effects.argtypes = [C.c_void_p, _pd, _pd]
     lib.pos_additional_effects.restype = C.c_int32
+    lib.pos_additional_effects_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.pos_additional_effects_noalloc.restype = C.c_int32
     lib.pos_history_enable.argtypes = [C.c_void_p, C.c_int32]
     lib.pos_history_enable.restype = C.c_int32
     lib.pos_history_drain.argtypes = [C.c_void_p, C.POINTER(HistoryRecord), C.c_int64, C.POINTER(C.c_int64)]
@@ -310,7 +314,7 @@ EXPORTED_SYMBOLS = (
     "pos_get_state", "pos_set_state", "pos_set_time_limit", "pos_set_historic_snapshot_period",
     "pos_get_counts", "pos_get_stream", "pos_get_options", "pos_get_launch_stats", "pos_tune",
     "pos_gravity_acceleration",
-    "pos_additional_effects", "pos_history_enable", "pos_history_drain", "pos_adjoint_record", "pos_adjoint",
+    "pos_additional_effects", "pos_additional_effects_noalloc", "pos_history_enable", "pos_history_drain", "pos_adjoint_record", "pos_adjoint",
     "pos_destroy",
 )
 

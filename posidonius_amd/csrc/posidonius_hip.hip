@@ -14,6 +14,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -304,6 +305,13 @@ __global__ __launch_bounds__(256) void permute_kernel(const double *__restrict__
 }
 
 }  // namespace
+
+// Scratch of pos_additional_effects_noalloc: the device-order accelerations
+// and spin derivatives of one evaluation, component-major, before they are put
+// into the caller's order.  Host code never passes this array's name to a
+// kernel (on the host it is a stand-in, not a device address); it uses the
+// address hipGetSymbolAddress gives per device (noalloc_scratch()).
+static __device__ __align__(256) double scratch[2 * 3 * (size_t)POS_NOALLOC_MAX_BODIES];
 
 // ===========================================================================
 // Host side
@@ -877,6 +885,26 @@ int ensure_graph(pos_ens *h) {
     return POS_OK;
 }
 
+// The device address of `scratch` on device `dev`, looked up once per device
+// with that device current (each GPU holds its own copy of the array).
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+int noalloc_scratch(int dev, double **out) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(POS_E_LIMIT, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(POS_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double *)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return POS_OK;
+}
+
 // One launch of effects_kernel per kernel variant present, writing device-order
 // results to d_a, d_l ([3][n_bodies] each).
 int launch_effects(pos_ens *h, double *d_a, double *d_l) {
@@ -1130,6 +1158,12 @@ int32_t pos_create(const pos_ensemble_desc *d, const pos_options *opts, pos_ens 
     }
     if ((rc = build_launch(h, o))) { pos_destroy(h); return rc; }
     if (hipStreamSynchronize(h->stream) != hipSuccess) { pos_destroy(h); return fail(POS_E_HIP, "synchronize failed"); }
+    // the no-allocation operator's device resources: the scratch array's
+    // address on this device and the body permutation
+    {
+        double *scratch_ptr = nullptr;
+        if ((rc = noalloc_scratch(h->device, &scratch_ptr)) || (rc = body_staging(h))) { pos_destroy(h); return rc; }
+    }
     h->eff_dev = eff;
     h->roles_dev = roles;
     h->host_dev = host_idx;
@@ -1392,6 +1426,28 @@ int32_t pos_additional_effects(pos_ens *h, double *out_acc, double *out_dLdt) {
     HIP_TRY(hipEventRecord(h->op_ev[1], h->stream));
     if ((rc = download_bodies(h, out_acc, d_a, 3)) || (rc = download_bodies(h, out_dLdt, d_l, 3))) return rc;
     return record_operator_time(h);
+}
+
+int32_t pos_additional_effects_noalloc(pos_ens *h, double *dev_acc, double *dev_dLdt) {
+    if (!h || !dev_acc || !dev_dLdt) return fail(POS_E_INVALID, "NULL argument");
+    const int nb = h->n_bodies;
+    if (nb > POS_NOALLOC_MAX_BODIES)
+        return fail(POS_E_LIMIT, "the ensemble has " + std::to_string(nb) + " bodies; the scratch holds " +
+                                     std::to_string(POS_NOALLOC_MAX_BODIES));
+    HIP_TRY(hipSetDevice(h->device));
+    double *s = nullptr;   // set up at pos_create; found here without allocating
+    int rc;
+    if ((rc = noalloc_scratch(h->device, &s))) return rc;
+    double *d_a = s, *d_l = s + 3 * (size_t)POS_NOALLOC_MAX_BODIES;
+    // every body belongs to one lane of one evaluated wavefront, so the kernels
+    // write every element the reordering reads: nothing needs clearing
+    if ((rc = launch_effects(h, d_a, d_l))) return rc;
+    const dim3 grid((nb + 255) / 256);
+    hipLaunchKernelGGL(unpermute_kernel, grid, dim3(256), 0, h->stream, d_a, h->d_body_perm, nb, 3, dev_acc);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(unpermute_kernel, grid, dim3(256), 0, h->stream, d_l, h->d_body_perm, nb, 3, dev_dLdt);
+    HIP_TRY(hipGetLastError());
+    return POS_OK;
 }
 
 int32_t pos_get_launch_stats(const pos_ens *h, pos_launch_stats *out) {
