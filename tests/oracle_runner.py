@@ -16,7 +16,9 @@ from posidonius_amd import abi, case
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+# POSIDONIUS_ORACLE_LIBRARY selects another build of the same sources (the
+# sanitizer build of tests/cpu/Makefile)
+LIB = os.environ.get("POSIDONIUS_ORACLE_LIBRARY") or os.path.join(ORACLE_DIR, "liboracle.so")
 GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
 
 

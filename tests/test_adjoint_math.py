@@ -23,8 +23,12 @@ HDR = os.path.join(os.path.dirname(HERE), "posidonius_amd", "csrc", "adjoint_mat
 
 @pytest.fixture(scope="module")
 def lib(tmp_path_factory):
-    out = str(tmp_path_factory.mktemp("adj") / "libadjcheck.so")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", SRC, "-o", out], check=True)
+    # POSIDONIUS_ADJCHECK_LIBRARY: a prebuilt library of the same source (the
+    # sanitizer build of tests/cpu/Makefile)
+    out = os.environ.get("POSIDONIUS_ADJCHECK_LIBRARY")
+    if not out:
+        out = str(tmp_path_factory.mktemp("adj") / "libadjcheck.so")
+        subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", SRC, "-o", out], check=True)
     lib = C.CDLL(out)
     d = C.POINTER(C.c_double)
     lib.adj_pair_forward.argtypes = [d, d, d, C.c_int, d]
