@@ -376,6 +376,33 @@ __device__ __forceinline__ void put3(WaveLds &lds, int slot, int lane, double x,
     lds.xb[slot][lane] = x; lds.xb[slot + 1][lane] = y; lds.xb[slot + 2][lane] = z;
 }
 
+// A reduction over the bodies of a lane's group in body order.  The terms of
+// FOLD_BATCH bodies are formed first (their LDS loads issue together and one
+// wait covers them), then folded in order.  Slots past the group's last body,
+// and the host's slot when SKIP_HOST, pass +0 to the fold.  Every reduction
+// folded this way starts at +0 and adds its terms, so it is never -0 (in
+// round-to-nearest x + y is -0 only when both x and y are); adding +0 to a
+// value that is not -0 leaves it unchanged bit for bit, so the result is the
+// sequential sum over the bodies it names.  The batch of 3 is the fastest of
+// 2, 3, 4, 6 and 8 measured on the benchmark ensemble (register pressure
+// grows with the batch).
+constexpr int FOLD_BATCH = 3;
+template <bool SKIP_HOST, class Term, class Fold>
+__device__ __forceinline__ void fold_group(const Lane &B, Term term, Fold fold) {
+    for (int j0 = 0; j0 < B.n; j0 += FOLD_BATCH) {
+        double t[FOLD_BATCH];
+#pragma unroll
+        for (int u = 0; u < FOLD_BATCH; ++u) {
+            const int j = j0 + u;
+            const bool use = j < B.n && !(SKIP_HOST && j == B.host);
+            const double v = term(use ? j : 0);   // body 0 keeps every index in the group
+            t[u] = use ? v : 0.;
+        }
+#pragma unroll
+        for (int u = 0; u < FOLD_BATCH; ++u) fold(t[u]);
+    }
+}
+
 // Ordered sums over the orbiting bodies of the group of the contribution rows
 // [0, C_COUNT), spread over the group's lanes: lane k sums rows k, k+n, ...
 // Every sum runs over the bodies in index order, so it is exactly the
@@ -386,10 +413,7 @@ __device__ __forceinline__ void host_reaction_sums(WaveLds &lds, const Lane &B) 
     const int hl = B.hl();
     for (int q = B.k; B.valid() && q < C_COUNT; q += B.n) {
         double acc = 0.;
-        for (int j = 0; j < B.n; ++j) {
-            if (j == B.host) continue;
-            acc = acc + lds.xb[q][B.gbase + j];
-        }
+        fold_group<true>(B, [&](int j) { return lds.xb[q][B.gbase + j]; }, [&](double t) { acc = acc + t; });
         lds.xb[q][hl] = acc;
     }
 }
@@ -413,6 +437,27 @@ constexpr int N_VARIANTS = 16;
 constexpr uint32_t EFF_VAR_SHIFT = 16;
 __host__ __device__ __forceinline__ int eff_variant(uint32_t eff) { return (int)((eff >> EFF_VAR_SHIFT) & 15u); }
 __host__ __device__ __forceinline__ int eff_gr_variant(uint32_t eff) { return eff_variant(eff) & 3; }
+
+// The effect and role tests of a lane for kernel variant V.  Under VAR_FAST
+// every effect is on and every orbiting body has every orbiting role, so the
+// tests are compile-time constants there (the same values the flags hold) and
+// the pair terms of an orbiting lane form one block the compiler can schedule
+// as a whole.
+template <int V>
+struct LaneTests {
+    bool tides, rf, gr, spin, tides_role, rf_role, gr_role;
+    __device__ __forceinline__ explicit LaneTests(const Lane &B) {
+        constexpr bool F = (V & VAR_FAST) != 0;
+        const bool orb = B.orbiting();
+        tides = F || B.tides();
+        rf = F || B.rf();
+        gr = F || B.gr();
+        spin = F || B.spin_on();
+        tides_role = F ? orb : B.tides_role();
+        rf_role = F ? orb : B.rf_role();
+        gr_role = F ? orb : B.gr_role();
+    }
+};
 
 // Chained order of general_relativity.rs (host, then the other bodies by
 // index): position of body k, and body at position c.
@@ -824,6 +869,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
     constexpr int GRV = V & 3;
     constexpr bool EXT = (V & VAR_EXT) != 0;
     static_assert(!H || ((V & VAR_FAST) != 0 && !EXT), "hoisted pair terms need fixed radii and factors");
+    const LaneTests<V> LT(B);
     const int lane = B.lane;
     const int hl = B.hl();
     const bool orbiting = B.orbiting();
@@ -929,7 +975,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         //      pair_ph, and a body that is not excited removes the host's
         //      entry, as the reference does (157-160). ----
         double sigma_host = lds.sigma[hl], sigma_pair = lds.sigma[lane];
-        if (EXT && B.tides_role() && (B.flags & (F_DYN_HOST | F_DYN))) {
+        if (EXT && LT.tides_role && (B.flags & (F_DYN_HOST | F_DYN))) {
             const double gm = mg_of(lds, hl) + mg_of(lds, lane);
             double q, e;
             perihelion_distance_and_eccentricity(gm, px, pv, q, e);
@@ -958,13 +1004,13 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         if (H) {
             os = geo.os;
             op = geo.op;
-        } else if (B.tides_role()) {
+        } else if (LT.tides_role) {
             os = divr(4.5 * p2(m) * p10(hR) * sigma_host, Rd7);
             op = divr(4.5 * p2(hm) * p10(R) * sigma_pair, Rd7);
         }
         // ---- flattening: orthogonal components (oblate_spheroid.rs:21-49) ----
         double fs = 0., fp = 0., ofs = 0., ofp = 0.;
-        if (B.rf_role()) {
+        if (LT.rf_role) {
             constexpr Rcp R6{6., 1. / 6.};
             fs = divr(m * lds.k2f[hl] * hs2 * p5(hR), R6);
             ofs = -6. * fs * rs_star / (hs2 * d5);
@@ -974,8 +1020,8 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         // A body the tidal loops skip adds nothing to the host's reaction sums:
         // its rows hold +0, which leaves an ordered sum that starts at +0
         // unchanged (such a sum is never -0).
-        if (B.tides() && !B.tides_role()) put3(lds, C_TF, lane, 0., 0., 0.);
-        if (B.tides_role()) {
+        if (LT.tides && !LT.tides_role) put3(lds, C_TF, lane, 0., 0., 0.);
+        if (LT.tides_role) {
             // radial component (constant_time_lag.rs:266-307)
             const double hm2 = hm * hm;
             const double pm2 = m * m;
@@ -1005,8 +1051,8 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
 #pragma unroll
             for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + dk[q];
         }
-        if (B.rf() && !B.rf_role()) put3(lds, C_RF, lane, 0., 0., 0.);
-        if (B.rf_role()) {
+        if (LT.rf && !LT.rf_role) put3(lds, C_RF, lane, 0., 0., 0.);
+        if (LT.rf_role) {
             // radial component (oblate_spheroid.rs:51-60) and force (83-97)
             const double radial = (H ? geo.m3d5 : -3. / d5) * (fp + fs)
                 + (H ? geo.c15 : divr(15., Rd7)) * (fs * rs_star * rs_star / hs2 + fp * rs_planet * rs_planet / s2);
@@ -1018,11 +1064,11 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
             a_add[1] = a_add[1] + B.inv_m * F1;
             a_add[2] = a_add[2] + B.inv_m * F2;
         }
-        if (B.spin_on()) {
+        if (LT.spin) {
             // torques (constant_time_lag.rs:171-204, tides/common.rs:223-261;
             // oblate_spheroid.rs:62-81, rotational_flattening/common.rs:165-201)
             double tdL[3] = {0., 0., 0.}, rdL[3] = {0., 0., 0.};
-            if (B.tides() && !B.tides_role()) {
+            if (LT.tides && !LT.tides_role) {
                 // The host's tidal torque sums over every other body, whatever
                 // its role (tides/common.rs:244-255, the test is on the host's
                 // effect), and a body the tidal loops skip keeps the zero
@@ -1032,7 +1078,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
                 const double nan = __builtin_nan("");
                 put3(lds, C_TN, lane, nan, nan, nan);
             }
-            if (B.tides_role()) {
+            if (LT.tides_role) {
                 const double cx = px[1] * pv[2] - px[2] * pv[1];
                 const double cy = px[2] * pv[0] - px[0] * pv[2];
                 const double cz = px[0] * pv[1] - px[1] * pv[0];
@@ -1045,8 +1091,8 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
                      -1.0 * (os * (d * hs[1] - divr(rs_star * px[1], Rd) - id * cy)),
                      -1.0 * (os * (d * hs[2] - divr(rs_star * px[2], Rd) - id * cz)));
             }
-            if (B.rf() && !B.rf_role()) put3(lds, C_RN, lane, 0., 0., 0.);   // its torque term is -1 * 0 * 0
-            if (B.rf_role()) {
+            if (LT.rf && !LT.rf_role) put3(lds, C_RN, lane, 0., 0., 0.);   // its torque term is -1 * 0 * 0
+            if (LT.rf_role) {
                 rdL[0] = -1.0 * (ofp * (px[1] * s[2] - px[2] * s[1]));
                 rdL[1] = -1.0 * (ofp * (px[2] * s[0] - px[0] * s[2]));
                 rdL[2] = -1.0 * (ofp * (px[0] * s[1] - px[1] * s[0]));
@@ -1058,13 +1104,13 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
 #pragma unroll
             for (int q = 0; q < 3; ++q) dLdt[q] = tdL[q] + rdL[q];
         }
-        if (GRV == GRV_KIDDER && B.gr() && !B.gr_role()) {   // skipped by the Kidder loops
+        if (GRV == GRV_KIDDER && LT.gr && !LT.gr_role) {   // skipped by the Kidder loops
             put3(lds, C_G1, lane, 0., 0., 0.);
             put3(lds, C_G2, lane, 0., 0., 0.);
             put3(lds, C_G3, lane, 0., 0., 0.);
             put3(lds, C_GL, lane, 0., 0., 0.);
         }
-        if (GRV == GRV_KIDDER && B.gr_role()) {
+        if (GRV == GRV_KIDDER && LT.gr_role) {
             // Kidder 1995 (general_relativity.rs:177-456)
             const Rcp Rhm{hm, lds.inv_m[hl]};
             const Rcp Rm{m, B.inv_m};
@@ -1172,10 +1218,10 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
     if (B.valid() && B.is_host()) {
         const double *col = &lds.xb[0][lane];   // this lane's column, row stride WAVE
         auto row = [&](int q) { return col[q * WAVE]; };
-        if (B.tides()) {   // tides/common.rs:312-345 and 223-261
+        if (LT.tides) {   // tides/common.rs:312-345 and 223-261
 #pragma unroll
             for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + -1.0 * B.inv_m * row(C_TF + q);
-            if (B.spin_on()) {
+            if (LT.spin) {
 #pragma unroll
                 for (int q = 0; q < 3; ++q) dLdt[q] = row(C_TN + q);
             }
@@ -1184,16 +1230,16 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
 #pragma unroll
             for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + dk[q];
         }
-        if (B.rf()) {      // rotational_flattening/common.rs:203-237 and 165-201
+        if (LT.rf) {      // rotational_flattening/common.rs:203-237 and 165-201
 #pragma unroll
             for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + -1.0 * B.inv_m * row(C_RF + q);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + (B.spin_on() ? row(C_RN + q) : 0.);
+            for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + (LT.spin ? row(C_RN + q) : 0.);
         } else {
 #pragma unroll
             for (int q = 0; q < 3; ++q) dLdt[q] = dLdt[q] + 0.;
         }
-        if (GRV == GRV_KIDDER && B.gr()) {      // general_relativity.rs:236-238, 295-297, 453-455, 312-315
+        if (GRV == GRV_KIDDER && LT.gr) {      // general_relativity.rs:236-238, 295-297, 453-455, 312-315
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 double g = -1.0 * row(C_G1 + q);
@@ -1213,14 +1259,14 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         double g[3];
         if (GRV == GRV_ANDERSON) gr_anderson1975(lds, B, g);
         else gr_newhall1983(lds, B, g);
-        if (B.valid() && B.gr()) {
+        if (B.valid() && LT.gr) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + g[q];
         }
     }
     // ---- wind (wind.rs:61-91), own body, added last ----
     double w[3] = {0., 0., 0.};
-    if (B.valid() && (B.flags & F_WIND) && B.spin_on() && (B.flags & F_WIND_ROLE)) {
+    if (B.valid() && (B.flags & F_WIND) && LT.spin && (B.flags & F_WIND_ROLE)) {
         const double wk = E.wind_k[B.body], wsat = E.wind_sat[B.body];
         const double s2 = lds.s2[lane];
         const double threshold = sqrt(s2);
@@ -1237,7 +1283,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
 template <int V>
 __device__ inline void integrate_velocity_dependent_forces(const DevEnsemble &E, WaveLds &lds, Lane &B, bool evolution) {
     const double h = 0.5 * B.dt;
-    const bool spin_on = B.spin_on();
+    const bool spin_on = LaneTests<V>(B).spin;
     // The final state of an iteration is v_orig + dv (and L_orig + dL), so only
     // the increments are carried from one iteration to the next; the previous
     // final state and the midpoint are recomputed from them, bit for bit.
@@ -1281,13 +1327,10 @@ __device__ inline void integrate_velocity_dependent_forces(const DevEnsemble &E,
             }
             wsync();
             double dv2 = 0., sv2 = 0., dL2 = 0., sL2 = 0.;
-            for (int j = 0; j < B.n; ++j) {
-                const int l = B.gbase + j;
-                dv2 = dv2 + lds.xb[0][l];
-                sv2 = sv2 + lds.xb[1][l];
-                dL2 = dL2 + lds.xb[2][l];
-                sL2 = sL2 + lds.xb[3][l];
-            }
+            fold_group<false>(B, [&](int j) { return lds.xb[0][B.gbase + j]; }, [&](double t) { dv2 = dv2 + t; });
+            fold_group<false>(B, [&](int j) { return lds.xb[1][B.gbase + j]; }, [&](double t) { sv2 = sv2 + t; });
+            fold_group<false>(B, [&](int j) { return lds.xb[2][B.gbase + j]; }, [&](double t) { dL2 = dL2 + t; });
+            fold_group<false>(B, [&](int j) { return lds.xb[3][B.gbase + j]; }, [&](double t) { sL2 = sL2 + t; });
             wsync();
             const bool ok = (spin_on && dv2 / sv2 < DBL_EPS_2 && dL2 / sL2 < DBL_EPS_2) ||
                             (!spin_on && dv2 / sv2 < DBL_EPS_2);
