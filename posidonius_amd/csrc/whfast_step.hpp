@@ -376,45 +376,62 @@ __device__ __forceinline__ void put3(WaveLds &lds, int slot, int lane, double x,
     lds.xb[slot][lane] = x; lds.xb[slot + 1][lane] = y; lds.xb[slot + 2][lane] = z;
 }
 
-// A reduction over the bodies of a lane's group in body order.  The terms of
-// FOLD_BATCH bodies are formed first (their LDS loads issue together and one
-// wait covers them), then folded in order.  Slots past the group's last body,
-// and the host's slot when SKIP_HOST, pass +0 to the fold.  Every reduction
-// folded this way starts at +0 and adds its terms, so it is never -0 (in
-// round-to-nearest x + y is -0 only when both x and y are); adding +0 to a
-// value that is not -0 leaves it unchanged bit for bit, so the result is the
-// sequential sum over the bodies it names.  The batch of 3 is the fastest of
-// 2, 3, 4, 6 and 8 measured on the benchmark ensemble (register pressure
-// grows with the batch).
+// R reductions over the bodies of a lane's group, each in body order.  The
+// terms of FOLD_BATCH bodies of every row are formed first (their LDS loads
+// issue together and one wait covers them), then folded row by row in body
+// order.  Slots past the group's last body, and the host's slot when
+// SKIP_HOST, fold +0.  Every reduction folded this way starts at +0 and adds
+// its terms, so it is never -0 (in round-to-nearest x + y is -0 only when
+// both x and y are); adding +0 to a value that is not -0 leaves it unchanged
+// bit for bit, so each result is the sequential sum over the bodies it names.
+// FOLD_BATCH and the rows per pass of the reaction sums (REACTION_ROWS) are
+// the fastest of the settings measured on the benchmark ensemble: the loads in
+// flight cost registers.
 constexpr int FOLD_BATCH = 3;
-template <bool SKIP_HOST, class Term, class Fold>
-__device__ __forceinline__ void fold_group(const Lane &B, Term term, Fold fold) {
+constexpr int REACTION_ROWS = 2;
+template <int R, bool SKIP_HOST, class Load>
+__device__ __forceinline__ void fold_rows(const Lane &B, double (&acc)[R], Load load) {
     for (int j0 = 0; j0 < B.n; j0 += FOLD_BATCH) {
-        double t[FOLD_BATCH];
+        double t[R][FOLD_BATCH];
 #pragma unroll
         for (int u = 0; u < FOLD_BATCH; ++u) {
             const int j = j0 + u;
             const bool use = j < B.n && !(SKIP_HOST && j == B.host);
-            const double v = term(use ? j : 0);   // body 0 keeps every index in the group
-            t[u] = use ? v : 0.;
+            const int l = B.gbase + (use ? j : 0);   // body 0 keeps every index in the group
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double v = load(r, l);
+                t[r][u] = use ? v : 0.;
+            }
         }
 #pragma unroll
-        for (int u = 0; u < FOLD_BATCH; ++u) fold(t[u]);
+        for (int u = 0; u < FOLD_BATCH; ++u)
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = acc[r] + t[r][u];
     }
 }
 
 // Ordered sums over the orbiting bodies of the group of the contribution rows
-// [0, C_COUNT), spread over the group's lanes: lane k sums rows k, k+n, ...
-// Every sum runs over the bodies in index order, so it is exactly the
-// sequential reference sum.  The host's column of the contribution rows is
-// never an input (the host contributes nothing), so each result is written
-// there; after the next wsync() the host reads its reactions in its own column.
+// [0, C_COUNT), spread over the group's lanes: lane k sums rows k, k+n, ...,
+// REACTION_ROWS of them per pass.  Every sum runs over the bodies in index
+// order, so it is exactly the sequential reference sum.  The host's column of
+// the contribution rows is never an input (the host contributes nothing), so
+// each result is written there; after the next wsync() the host reads its
+// reactions in its own column.
 __device__ __forceinline__ void host_reaction_sums(WaveLds &lds, const Lane &B) {
     const int hl = B.hl();
-    for (int q = B.k; B.valid() && q < C_COUNT; q += B.n) {
-        double acc = 0.;
-        fold_group<true>(B, [&](int j) { return lds.xb[q][B.gbase + j]; }, [&](double t) { acc = acc + t; });
-        lds.xb[q][hl] = acc;
+    for (int q0 = B.k; B.valid() && q0 < C_COUNT; q0 += REACTION_ROWS * B.n) {
+        int q[REACTION_ROWS];
+        double acc[REACTION_ROWS];
+#pragma unroll
+        for (int r = 0; r < REACTION_ROWS; ++r) {
+            q[r] = q0 + r * B.n < C_COUNT ? q0 + r * B.n : q0;   // a row past the last reads row q0, unused
+            acc[r] = 0.;
+        }
+        fold_rows<REACTION_ROWS, true>(B, acc, [&](int r, int l) { return lds.xb[q[r]][l]; });
+#pragma unroll
+        for (int r = 0; r < REACTION_ROWS; ++r)
+            if (q0 + r * B.n < C_COUNT) lds.xb[q[r]][hl] = acc[r];
     }
 }
 
@@ -1326,11 +1343,9 @@ __device__ inline void integrate_velocity_dependent_forces(const DevEnsemble &E,
                 lds.xb[3][B.lane] = fL2;
             }
             wsync();
-            double dv2 = 0., sv2 = 0., dL2 = 0., sL2 = 0.;
-            fold_group<false>(B, [&](int j) { return lds.xb[0][B.gbase + j]; }, [&](double t) { dv2 = dv2 + t; });
-            fold_group<false>(B, [&](int j) { return lds.xb[1][B.gbase + j]; }, [&](double t) { sv2 = sv2 + t; });
-            fold_group<false>(B, [&](int j) { return lds.xb[2][B.gbase + j]; }, [&](double t) { dL2 = dL2 + t; });
-            fold_group<false>(B, [&](int j) { return lds.xb[3][B.gbase + j]; }, [&](double t) { sL2 = sL2 + t; });
+            double sums[4] = {0., 0., 0., 0.};
+            fold_rows<4, false>(B, sums, [&](int r, int l) { return lds.xb[r][l]; });
+            const double dv2 = sums[0], sv2 = sums[1], dL2 = sums[2], sL2 = sums[3];
             wsync();
             const bool ok = (spin_on && dv2 / sv2 < DBL_EPS_2 && dL2 / sL2 < DBL_EPS_2) ||
                             (!spin_on && dv2 / sv2 < DBL_EPS_2);
