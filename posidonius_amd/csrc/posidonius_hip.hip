@@ -1502,14 +1502,14 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
     // next power of two / exactly N lanes per size class
     std::vector<int> groups = {0, -1};
     {
-        // exactly N lanes with sizes mixed in a wavefront, when the size
-        // classes leave more than a tenth of the lanes idle (it measures the
-        // same as the size classes when they fill the lanes)
+        // exactly N lanes with sizes mixed in a wavefront, whenever the
+        // systems differ in size: the packing needs fewer wavefronts, and
+        // whether that pays depends on how the last round of wavefronts
+        // falls on the SIMDs (one wavefront per SIMD runs faster than two),
+        // which only a measurement shows
         std::map<int, int64_t> per_size;
-        int64_t lanes = 0, waves = 0;
-        for (int n : h->n_per_sys_dev) { ++per_size[n]; lanes += n; }
-        for (const auto &c : per_size) waves += (c.second + 64 / c.first - 1) / (64 / c.first);
-        if (10 * lanes < 9 * 64 * waves) groups.push_back(-2);
+        for (int n : h->n_per_sys_dev) ++per_size[n];
+        if (per_size.size() > 1) groups.push_back(-2);
     }
     for (int g = max_n; g <= 16; ++g)
         if (64 % g != 0) groups.push_back(g);   // exact packing for sizes that do not divide 64
