@@ -187,7 +187,10 @@ void whfast_steps_kernel(DevEnsemble E, int n_steps, int wave_begin, int wave_en
         load_lane<V>(E, lds, B, wave);
         const TapeRef none{nullptr, 0};
         for (int step = 0; step < n_steps; ++step) {
-            if (B.status & FROZEN) break;   // uniform within the group
+            // uniform within the group, not the wavefront: the other groups keep
+            // calling wsync(), which is correct only because the workgroup is
+            // this one wavefront (__launch_bounds__(64), launched with 64 threads)
+            if (B.status & FROZEN) break;
             iterate<false, V>(E, lds, B, none, 0);
         }
         store_lane(E, lds, B);
@@ -207,7 +210,10 @@ void whfast_record_kernel(DevEnsemble E, int n_steps, TapeRef T, int wave_begin,
         Lane B;
         load_lane<V>(E, lds, B, wave);
         for (int step = 0; step < n_steps; ++step) {
-            if (B.status & FROZEN) break;   // uniform within the group
+            // uniform within the group, not the wavefront: the other groups keep
+            // calling wsync(), which is correct only because the workgroup is
+            // this one wavefront (__launch_bounds__(64), launched with 64 threads)
+            if (B.status & FROZEN) break;
             iterate<true, V>(E, lds, B, T, step);
         }
         store_lane(E, lds, B);
@@ -222,7 +228,9 @@ __global__ __launch_bounds__(64) void adjoint_segment_kernel(AdjArgs Ar) {
         AdjLane A;
         adjoint_load(Ar, L, A, wave);
         for (int j = Ar.seg_len - 1; j >= 0; --j) {
-            if (Ar.seg_start + j >= A.done) continue;   // uniform within the group
+            // uniform within the group, not the wavefront: correct only because
+            // the workgroup is this one wavefront (see whfast_steps_kernel)
+            if (Ar.seg_start + j >= A.done) continue;
             adjoint_step(Ar, L, A, j);
         }
         adjoint_store(Ar, A);
