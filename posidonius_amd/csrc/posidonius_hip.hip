@@ -1468,40 +1468,51 @@ int32_t pos_tune(pos_ens *h, int32_t probe_steps, pos_options *chosen) {
     if (!h) return fail(POS_E_INVALID, "handle is NULL");
     if (probe_steps <= 0) probe_steps = 32;
     HIP_TRY(hipSetDevice(h->device));
-    // snapshot the mutable state so that probing leaves it untouched
+    // snapshot the mutable state so that probing leaves it untouched: one
+    // device buffer, filled and restored with device-to-device copies on
+    // the handle's stream (the GPU does not wait on host copies between the
+    // probes, nor before the caller's first steps)
     const int ns = h->n_systems, nb = h->n_bodies;
-    std::vector<double> sx(3 * (size_t)nb), sv(3 * (size_t)nb), sL(3 * (size_t)nb), ss(3 * (size_t)nb), sev(3 * (size_t)nb),
-        seL(3 * (size_t)nb), sacc(3 * (size_t)nb), sR(nb), srg2(nb), sI(nb), st(ns), slh(ns), slag(nb), smig(nb),
-        spair(nb);
-    std::vector<long long> sit(ns);
-    std::vector<uint32_t> sst(ns);
-    std::vector<int> stw(ns), sleft(nb);
     int rc;
     // the probes are not part of the caller's run: no historic records, and
     // the launch counters are restored at the end
     HistoryDetach detached(h);
     const pos_launch_stats stats0 = h->stats;
     const DevEnsemble &E = h->E;
-    if ((rc = download(h, sx, E.x, 3 * (size_t)nb)) || (rc = download(h, sv, E.v, 3 * (size_t)nb)) ||
-        (rc = download(h, sL, E.L, 3 * (size_t)nb)) || (rc = download(h, ss, E.spin, 3 * (size_t)nb)) ||
-        (rc = download(h, sev, E.err_v, 3 * (size_t)nb)) || (rc = download(h, seL, E.err_L, 3 * (size_t)nb)) ||
-        (rc = download(h, sacc, E.acc, 3 * (size_t)nb)) || (rc = download(h, sR, E.radius, nb)) ||
-        (rc = download(h, srg2, E.rg2, nb)) || (rc = download(h, sI, E.inertia, nb)) || (rc = download(h, st, E.t, ns)) ||
-        (rc = download(h, slh, E.last_hist, ns)) || (rc = download(h, sit, E.iteration, ns)) ||
-        (rc = download(h, sst, E.status, ns)) || (rc = download(h, stw, E.timestep_warning, ns)) ||
-        (rc = download(h, sleft, E.left_index, nb)) || (rc = download(h, slag, E.lag, nb)) ||
-        (rc = download(h, smig, E.disk_migration, nb)) || (rc = download(h, spair, E.pair_ph, nb)))
-        return rc;
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    struct Field { void *p; size_t bytes; };
+    const Field fields[] = {
+        {E.x, 3 * (size_t)nb * sizeof(double)}, {E.v, 3 * (size_t)nb * sizeof(double)},
+        {E.L, 3 * (size_t)nb * sizeof(double)}, {E.spin, 3 * (size_t)nb * sizeof(double)},
+        {E.err_v, 3 * (size_t)nb * sizeof(double)}, {E.err_L, 3 * (size_t)nb * sizeof(double)},
+        {E.acc, 3 * (size_t)nb * sizeof(double)}, {E.radius, (size_t)nb * sizeof(double)},
+        {E.rg2, (size_t)nb * sizeof(double)}, {E.inertia, (size_t)nb * sizeof(double)},
+        {E.t, (size_t)ns * sizeof(double)}, {E.last_hist, (size_t)ns * sizeof(double)},
+        {E.iteration, (size_t)ns * sizeof(long long)}, {E.status, (size_t)ns * sizeof(uint32_t)},
+        {E.timestep_warning, (size_t)ns * sizeof(int)}, {E.left_index, (size_t)nb * sizeof(int)},
+        {E.lag, (size_t)nb * sizeof(double)}, {E.disk_migration, (size_t)nb * sizeof(double)},
+        {E.pair_ph, (size_t)nb * sizeof(double)}};
+    size_t total = 0;
+    for (const Field &f : fields) total += (f.bytes + 255) / 256 * 256;
+    struct Snapshot {   // freed on every return path
+        char *p = nullptr;
+        ~Snapshot() { if (p) (void)hipFree(p); }
+    } snap;
+    HIP_TRY(hipMalloc((void **)&snap.p, total));
+    auto copy_all = [&](bool save) -> int {
+        size_t off = 0;
+        for (const Field &f : fields) {
+            if (f.bytes && f.p) {
+                if (save) HIP_TRY(hipMemcpyAsync(snap.p + off, f.p, f.bytes, hipMemcpyDeviceToDevice, h->stream));
+                else HIP_TRY(hipMemcpyAsync(f.p, snap.p + off, f.bytes, hipMemcpyDeviceToDevice, h->stream));
+            }
+            off += (f.bytes + 255) / 256 * 256;
+        }
+        return POS_OK;
+    };
+    if ((rc = copy_all(true))) return rc;
     auto restore = [&]() -> int {
         int r;
-        if ((r = upload(h, E.x, sx)) || (r = upload(h, E.v, sv)) || (r = upload(h, E.L, sL)) || (r = upload(h, E.spin, ss)) ||
-            (r = upload(h, E.err_v, sev)) || (r = upload(h, E.err_L, seL)) || (r = upload(h, E.acc, sacc)) ||
-            (r = upload(h, E.radius, sR)) || (r = upload(h, E.rg2, srg2)) || (r = upload(h, E.inertia, sI)) ||
-            (r = upload(h, E.t, st)) || (r = upload(h, E.last_hist, slh)) || (r = upload(h, E.iteration, sit)) ||
-            (r = upload(h, E.status, sst)) || (r = upload(h, E.timestep_warning, stw)) || (r = upload(h, E.left_index, sleft)) ||
-            (r = upload(h, E.lag, slag)) || (r = upload(h, E.disk_migration, smig)) || (r = upload(h, E.pair_ph, spair)))
-            return r;
+        if ((r = copy_all(false))) return r;
         HIP_TRY(hipStreamSynchronize(h->stream));
         return POS_OK;
     };
