@@ -211,6 +211,64 @@ def operator_rates(ens, n_sys, calls=5):
     return out
 
 
+GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
+# The effect sets of the reference's operator benchmark group
+# calculate_additional_effects (benches/simulation.rs:138-221; its Kaula
+# universe is out of scope here), by the ABI's effect bits.
+EFFECT_SETS = (("all", 0b111111), ("constant_time_lag_tides", 1 << 0), ("rotational_flattening", 1 << 1),
+               ("general_relativity", 1 << 2), ("disk", 1 << 3), ("wind", 1 << 4), ("evolution", 1 << 5),
+               ("none", 0))
+
+
+def benchmark_universe():
+    """The universe of the reference's operator benchmark
+    (benches/simulation.rs:14-46): a solar-like star with a protoplanetary
+    disk, GalletBolmont2017 evolution and Kidder 1995 GR, and the common
+    four-planet configuration, at t = 0.  The committed disk fixture holds that
+    star and those planets with Baraffe1998 evolution; the star's evolution
+    and its evolution table are taken from the GalletBolmont2017 fixture."""
+    import copy
+    from posidonius_amd.case import read_case
+    with open(os.path.join(GOLDEN_DIR, "test_disk-enabled_disk.json")) as f:
+        case = copy.deepcopy(json.load(f)["case"])
+    with open(os.path.join(GOLDEN_DIR, "test_evolution-solar_like_galletbolmont2017.json")) as f:
+        gb = json.load(f)["case"]
+    star, gb_star = case["universe"]["particles"][0], gb["universe"]["particles"][0]
+    for key in ("evolution", "radius", "radius_of_gyration_2", "moment_of_inertia"):
+        star[key] = gb_star[key]
+    case["universe"]["particles_evolvers"][0] = gb["universe"]["particles_evolvers"][0]
+    return read_case(case)
+
+
+def effect_operator_rates(n_copies, calls=3):
+    """pos_additional_effects over n_copies of the benchmark universe, once per
+    effect set of the reference's benchmark group: the evaluation kernels'
+    device time (HIP events on the library's stream), as kernel_ms and
+    systems per second."""
+    from posidonius_amd.case import EnsembleSpec
+    from posidonius_amd.ensemble import Ensemble
+    spec = benchmark_universe()
+    while spec.n_systems < n_copies:
+        spec = EnsembleSpec.concat([spec, spec])
+    out = {"workload": f"{spec.n_systems} copies of the benchmark universe (star with disk + 4 planets)",
+           "note": "the spin derivatives are evaluated as the step evaluates them (only when spins are "
+                   "integrated: tides, flattening, evolution or Kidder GR); the reference's benchmark asks "
+                   "for them in every set, which differs for the wind set alone"}
+    for name, mask in EFFECT_SETS:
+        s = spec.copy()
+        s.system["effects"] = s.system["effects"].copy()
+        s.system["effects"][:] = mask
+        with Ensemble(s, mapping="packed", group_size=-1) as ens:
+            ens.additional_effects()
+            kernel = []
+            for _ in range(calls):
+                ens.additional_effects()
+                kernel.append(ens.launch_stats()["last_operator_kernel_ms"])
+        k_ms = sum(kernel) / len(kernel)
+        out[name] = {"kernel_ms": k_ms, "kernel_systems_per_s": s.n_systems / (k_ms * 1e-3)}
+    return out
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -307,6 +365,10 @@ def main():
         log(f"operators (kernels): gravity {operators['gravity']['kernel_systems_per_s']:.4g} systems/s, "
             f"additional effects {operators['additional_effects']['kernel_systems_per_s']:.4g} systems/s")
     ens.close()
+    if operators is not None:
+        log("additional effects per effect set, on the reference benchmark's universe")
+        operators["per_effect"] = effect_operator_rates(65536)
+        log(", ".join(f"{k} {v['kernel_ms']:.3f} ms" for k, v in operators["per_effect"].items() if isinstance(v, dict)))
     value = n_sys * args.steps / wall
     # the dominant kernel: one launch per variant present (one here), each of
     # launch_steps steps; the average launch is the timed region / launches

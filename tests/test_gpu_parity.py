@@ -131,6 +131,29 @@ def test_operators_against_oracle(Ensemble, oracle, mapping, group_size, graph):
     np.testing.assert_array_equal(e_dl, r_edl)
 
 
+def test_operator_per_effect_set_on_the_benchmark_universe(Ensemble, oracle):
+    """pos_additional_effects on the universe of the reference's operator
+    benchmark (benches/simulation.rs:14-46, 138-221: a star with a disk and
+    GalletBolmont2017 evolution, four planets), once per effect set of that
+    benchmark, 8 copies each, against the oracle bit for bit; bench.py times
+    the same sets."""
+    import bench
+    base = bench.benchmark_universe()
+    spec = EnsembleSpec.concat([base] * 8)
+    for name, mask in bench.EFFECT_SETS:
+        s = spec.copy()
+        s.system["effects"] = s.system["effects"].copy()
+        s.system["effects"][:] = mask
+        log(f"effect set {name}")
+        with Ensemble(s, mapping="packed", group_size=-1) as ens:
+            acc, dl = ens.additional_effects()
+        r_acc, r_dl = oracle.additional_effects(s)
+        np.testing.assert_array_equal(acc, r_acc, err_msg=name)
+        np.testing.assert_array_equal(dl, r_dl, err_msg=name)
+        if mask & (abi.EFFECT_TIDES | abi.EFFECT_DISK | abi.EFFECT_ROTATIONAL_FLATTENING | abi.EFFECT_GENERAL_RELATIVITY):
+            assert np.abs(acc).max() > 0, name
+
+
 def test_resume_equals_continuous_run(Ensemble, oracle):
     """get_state -> new ensemble -> continue gives the same bits as one run
     (checkpoint/resume, src/integrator/output.rs:28-54, 178-203)."""
