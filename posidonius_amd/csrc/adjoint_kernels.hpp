@@ -1,5 +1,5 @@
 // adjoint_kernels.hpp -- reverse sweep of the discrete adjoint of the WHFast
-// democratic-heliocentric step (tides, rotational flattening, Kidder 1995 GR,
+// democratic-heliocentric step (tides, rotational flattening, Kidder 1995 GR, wind,
 // spin), one segment of recorded steps at a time.
 //
 // Mapping: the forward step's (one lane per body, one lane group per system,
@@ -66,8 +66,9 @@ struct AdjArgs {
 
 struct AdjLane {
     int lane, gbase, k, n, host, sys, body;
-    bool valid, is_host, tides, flat, gr, spin;
+    bool valid, is_host, tides, flat, gr, spin, wind;
     double dt, m, I, R, sg, k2, kf, mh, Ih, M;
+    double wk, wsat, wshape;   // this body's wind k factor, saturation and shape factor
     long long done;
     double lx[3], lv[3], lL[3];
     double gs, gk2, gkf;
@@ -153,6 +154,16 @@ __device__ inline void effects_vjp_pass(AdjLds &L, const AdjLane &A, const posad
             const V3 p = gsum3(L, A, 24, true);
             pg[0] = p.x; pg[1] = p.y; pg[2] = p.z;
         }
+    }
+    if (A.wind) {
+        // this body's own wind torque (wind.rs:61-91), -k W f shape with
+        // f = wsat^2 once |W| reaches the saturation and |W|^2 below it:
+        // its transpose Jacobian with respect to W, then W = L / I
+        const V3 W = in.Wp;
+        const double s2 = posadj::dot(W, W);
+        const V3 dW = sqrt(s2) >= A.wsat ? (-A.wk * A.wsat * A.wsat * A.wshape) * mu_L
+                                         : (-A.wk * A.wshape) * (s2 * mu_L + (2. * posadj::dot(W, mu_L)) * W);
+        gL = gL + (1. / A.I) * dW;
     }
     wsync();
 }
@@ -392,6 +403,12 @@ __device__ inline void adjoint_load(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     A.sg = E.sigma[b];
     A.k2 = E.k2[b];
     A.kf = E.k2f[b];
+    // the wind torque enters dL/dt only where spins are integrated, as in
+    // the forward step (whfast.rs:259-262)
+    A.wind = A.valid && A.spin && (eff & POS_EFFECT_WIND) && (E.roles[b] & POS_ROLE_WIND);
+    A.wk = A.wind ? E.wind_k[b] : 0.;
+    A.wsat = A.wind ? E.wind_sat[b] : 0.;
+    A.wshape = A.wind ? sqrt(A.R / R_SUN * 1. / A.m) : 0.;
     L.m[lane] = A.m;
     L.I[lane] = A.I;
     L.R[lane] = A.R;

@@ -1695,8 +1695,8 @@ int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
         const std::string where = "system " + std::to_string(h->sys_perm[ds]) + ": ";
         if (h->coord_dev[ds] != POS_COORD_DEMOCRATIC_HELIOCENTRIC)
             return fail(POS_E_UNSUPPORTED, where + "the adjoint supports democratic-heliocentric coordinates only");
-        if (h->eff_dev[ds] & (POS_EFFECT_WIND | POS_EFFECT_DISK | POS_EFFECT_EVOLUTION))
-            return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support wind, disk or evolution");
+        if (h->eff_dev[ds] & (POS_EFFECT_DISK | POS_EFFECT_EVOLUTION))
+            return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support the disk or evolution");
         if ((h->variant_dev[ds] & 3) != GRV_KIDDER)
             return fail(POS_E_UNSUPPORTED, where + "the adjoint supports Kidder1995 general relativity only");
         if (h->variant_dev[ds] & VAR_EXT)
