@@ -101,6 +101,8 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
     B.status = E.status[s];
     if (!valid) B.status |= POS_STATUS_COMPLETED;   // idle lanes never iterate
     B.tw = E.timestep_warning[s];
+    B.seen_status = B.status;   // per system: what the group agrees on at load
+    B.seen_tw = B.tw;
     B.left_index = E.left_index[b];
     B.hcount = E.hist ? E.hist_count[s] : 0;
     const int nb = E.n_bodies;

@@ -245,6 +245,8 @@ struct Lane {
     int lane, gbase, k, n, host, coord, sys, body;
     uint32_t flags, status;
     int tw, evo, left_index, hcount;
+    uint32_t seen_status;   // the status and warning count the group last agreed on
+    int seen_tw;
     long long iteration;
     double t, dt, last_hist;
     double hist_period, time_limit;   // per-system constants, read once per launch
@@ -1858,8 +1860,13 @@ __device__ inline void gravity(const DevEnsemble &E, WaveLds &lds, Lane &B) {
 }
 
 // Status bits raised by one lane (gravity checks, zero inertia, Kepler) are
-// OR-ed over the group so that every lane agrees.
+// OR-ed over the group so that every lane agrees, and the time-step warning
+// count is the group's largest.  Both change only on rare events: when no
+// active lane of the wavefront holds a status or count other than the one
+// its group agreed on last (a wavefront-wide vote), every lane already holds
+// the group's OR and maximum, and the exchange is skipped.
 __device__ inline void merge_status(WaveLds &lds, Lane &B) {
+    if (!__any(B.status != B.seen_status || B.tw != B.seen_tw)) return;
     lds.xb[0][B.lane] = __builtin_bit_cast(double, (unsigned long long)B.status);
     lds.xb[1][B.lane] = (double)B.tw;
     wsync();
@@ -1872,6 +1879,8 @@ __device__ inline void merge_status(WaveLds &lds, Lane &B) {
     wsync();
     B.status = st;
     B.tw = tw;
+    B.seen_status = st;
+    B.seen_tw = tw;
 }
 
 // ===========================================================================
