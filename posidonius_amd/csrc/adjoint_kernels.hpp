@@ -250,6 +250,13 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
         v1 = tape3(T, row, TP_V1, b); L1 = tape3(T, row, TP_L1, b);
         X1 = *T.at(row, TP_XK1, b); X2 = *T.at(row, TP_XK2, b);
         x2 = tape3(T, row + 1, TP_X, b); v2 = tape3(T, row + 1, TP_V, b); L2 = tape3(T, row + 1, TP_L, b);
+        // this step's radius and moment of inertia (they follow the evolution
+        // tables in time and depend on no state, so they carry no derivative)
+        A.R = *T.at(row, TP_R, b);
+        A.I = *T.at(row, TP_I, b);
+        L.R[lane] = A.R;
+        L.I[lane] = A.I;
+        if (A.wind) A.wshape = sqrt(A.R / R_SUN * 1. / A.m);
     }
     const double h = 0.5 * A.dt, dt = A.dt;
     const double mu = posadj::G_K2 * A.mh;
@@ -259,6 +266,7 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     // ---------------- recompute the drift from (x0, v1) ----------------
     if (A.valid) { put3(L, 0, lane, x0); put3(L, 3, lane, v1); }
     wsync();
+    A.Ih = L.I[hl];   // the host's moment of inertia of this step
     const V3 Xc = (1. / A.M) * gsum3m(L, A, 0, false);
     const V3 V = (1. / A.M) * gsum3m(L, A, 3, false);
     const V3 Q0 = orb ? x0 - get3(L, 0, hl) : z;

@@ -1697,15 +1697,12 @@ int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
         const std::string where = "system " + std::to_string(h->sys_perm[ds]) + ": ";
         if (h->coord_dev[ds] != POS_COORD_DEMOCRATIC_HELIOCENTRIC)
             return fail(POS_E_UNSUPPORTED, where + "the adjoint supports democratic-heliocentric coordinates only");
-        if (h->eff_dev[ds] & (POS_EFFECT_DISK | POS_EFFECT_EVOLUTION))
-            return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support the disk or evolution");
+        if (h->eff_dev[ds] & POS_EFFECT_DISK)
+            return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support the disk");
         if ((h->variant_dev[ds] & 3) != GRV_KIDDER)
             return fail(POS_E_UNSUPPORTED, where + "the adjoint supports Kidder1995 general relativity only");
         if (h->variant_dev[ds] & VAR_EXT)
             return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support evolving dissipation");
-        for (int b = h->body_offset_dev[ds]; b < h->body_offset_dev[ds + 1]; ++b)
-            if (h->evo_dev[b] != POS_EVO_NON_EVOLVING)
-                return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support evolving bodies");
         const uint32_t eff = h->eff_dev[ds];
         for (int b = h->body_offset_dev[ds]; b < h->body_offset_dev[ds + 1]; ++b) {
             if (b - h->body_offset_dev[ds] == h->host_dev[ds]) continue;

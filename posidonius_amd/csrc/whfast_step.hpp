@@ -1959,8 +1959,8 @@ __device__ inline void write_history_record(const DevEnsemble &E, WaveLds &lds, 
 // (v1, L1) and the universal anomalies of the two Kepler drifts.  Row
 // n_steps holds the state after the last step in its x, v, L fields.
 // ===========================================================================
-constexpr int TAPE_FIELDS = 17;
-enum TapeField { TP_X = 0, TP_V = 3, TP_L = 6, TP_V1 = 9, TP_L1 = 12, TP_XK1 = 15, TP_XK2 = 16 };
+constexpr int TAPE_FIELDS = 19;
+enum TapeField { TP_X = 0, TP_V = 3, TP_L = 6, TP_V1 = 9, TP_L1 = 12, TP_XK1 = 15, TP_XK2 = 16, TP_R = 17, TP_I = 18 };
 
 struct TapeRef {
     double *base;      // [rows][TAPE_FIELDS][n_bodies]
@@ -2003,6 +2003,10 @@ __device__ inline void iterate(const DevEnsemble &E, WaveLds &lds, Lane &B, cons
     if (REC && B.valid()) {
         tape_put3(T, row, TP_V1, B.body, B.v);
         tape_put3(T, row, TP_L1, B.body, B.L);
+        // the radius and moment of inertia both kicks of the step use (the
+        // evolution tables are evaluated at the step's time in the first)
+        *T.at(row, TP_R, B.body) = lds.R[B.lane];
+        *T.at(row, TP_I, B.body) = lds.I[B.lane];
     }
     inertial_to_alternative(lds, B);
     kepler_steps(lds, B, half_dt);

@@ -158,6 +158,79 @@ def test_adjoint_with_wind_equals_tangent_linear(Ensemble, oracle):
                 assert rel.max() < 1e-6, (blk, k, rel.max(), lhs[:3], rhs[:3])
 
 
+def evolving_workload(n_systems, steps):
+    """The adjoint workload with evolving stars.  The stars follow synthetic
+    evolution tables over the run's own span (the reference's stellar tables
+    barely move over a few hundred short steps): the even systems' stars are
+    Baraffe1998 (radius only), the odd ones Baraffe2015 (radius and radius
+    of gyration), the radius growing by 5-11% and the radius of gyration
+    moving by up to 10% within the run.  Without the evolution the oracle's
+    tangents of this run differ by up to 30%, so an adjoint that kept the
+    initial radii and moments of inertia would miss by orders of magnitude."""
+    from posidonius_amd import abi
+    from posidonius_amd.case import TABLE_FIELDS
+    spec = gen.config4(n_systems)
+    B = spec.body
+    nb = spec.n_bodies
+    spec.system["effects"] = spec.system["effects"] | abi.EFFECT_EVOLUTION
+    span = 1.1 * steps * float(np.max(spec.system["time_step"]))
+    evo = np.zeros(nb, dtype=np.int32)
+    offset = np.zeros(nb + 1, dtype=np.int32)
+    tabs = {k: [] for k in TABLE_FIELDS}
+    for b in range(nb):
+        rows = 0
+        s = np.searchsorted(spec.body_offset, b, side="right") - 1
+        if b == spec.body_offset[s]:
+            evo[b] = abi.EVO_BARAFFE1998 if s % 2 == 0 else abi.EVO_BARAFFE2015
+            t = np.array([0., span / 3, 2 * span / 3, span])
+            tabs["evolver_time"].append(t)
+            tabs["evolver_radius"].append(B["radius"][b] * np.array([1., 1.04, 1.07, 1.12]))
+            tabs["evolver_radius_of_gyration_2"].append(B["radius_of_gyration_2"][b] * np.array([1., 1.05, 0.97, 1.1]))
+            tabs["evolver_love_number"].append(np.full(4, B["tides_love_number"][b]))
+            tabs["evolver_inverse_tidal_q_factor"].append(np.zeros(4))
+            rows = 4
+        offset[b + 1] = offset[b] + rows
+    B["evolution_type"] = evo
+    B["evolver_left_index"] = np.zeros(nb, dtype=np.int32)
+    spec.evolver_offset = offset
+    spec.tables = {k: np.concatenate(v) for k, v in tabs.items()}
+    return spec
+
+
+def test_adjoint_with_evolution_equals_tangent_linear(Ensemble, oracle):
+    """Evolving stars: the reverse sweep takes each step's radius and moment
+    of inertia from the tape; against the tangent-linear run, every seed
+    block and direction to 1e-6 relative."""
+    steps, segment = 300, 100
+    spec = evolving_workload(16, steps)
+    rng = np.random.default_rng(29)
+    d = directions(spec, rng)
+    ref, tan = oracle.tangent(spec, steps, d, threads=max(1, min(16, os.cpu_count() or 1)))
+    h = spec.body_offset[:-1]
+    grown = ref["radius"][h] / spec.body["radius"][h]
+    assert grown.min() > 1.03, grown.min()
+    with Ensemble(spec, mapping="packed", steps_per_launch=64) as ens:
+        ens.adjoint_record(steps, segment_steps=segment)
+        fin = ens.get_state()
+        for k in BLOCKS + ("radius", "moment_of_inertia"):
+            np.testing.assert_array_equal(fin[k], ref[k], err_msg=f"recorded forward run differs in {k}")
+        for blk in BLOCKS:
+            w = rng.standard_normal((spec.n_bodies, 3)) / np.abs(fin[blk]).max(axis=1, keepdims=True)
+            g = ens.adjoint(**{blk: w})
+            tkey = {"position": "x", "velocity": "v", "angular_momentum": "L"}[blk]
+            for k in range(8):
+                lhs = per_system(spec, w * tan[tkey][k])
+                rhs = (per_system(spec, g["position"] * d["x"][k]) + per_system(spec, g["velocity"] * d["v"][k])
+                       + per_system(spec, g["angular_momentum"] * d["L"][k])
+                       + per_system(spec, g["dissipation_factor"] * d["sigma"][k])
+                       + per_system(spec, g["love_number"] * d["k2"][k])
+                       + per_system(spec, g["fluid_love_number"] * d["k2f"][k]))
+                scale = np.maximum(np.abs(lhs), np.abs(rhs))
+                rel = np.abs(lhs - rhs) / np.where(scale > 0, scale, 1.)
+                print(f"evolution: seed {blk:17s} dir {k}: max rel {rel.max():.2e}", flush=True)
+                assert rel.max() < 1e-6, (blk, k, rel.max(), lhs[:3], rhs[:3])
+
+
 def test_adjoint_is_the_same_for_every_launch_mapping(Ensemble):
     """Systems of 2-9 bodies: the recorded run and the adjoint give the same
     bits whether each size has its own wavefronts (exactly N or the next power
