@@ -51,6 +51,7 @@ constexpr int ADJ_MAX_PASSES = 12;
 
 struct AdjLds {
     double m[WAVE], I[WAVE], R[WAVE], sg[WAVE], k2[WAVE], kf[WAVE];
+    double eta[WAVE];   // Jacobi: the mass of the host and the bodies up to this one in chain order
     double xb[ADJ_ROWS][WAVE];
 };
 
@@ -66,7 +67,9 @@ struct AdjArgs {
 
 struct AdjLane {
     int lane, gbase, k, n, host, sys, body;
-    bool valid, is_host, tides, flat, gr, spin, wind;
+    bool valid, is_host, tides, flat, gr, spin, wind, jacobi;
+    int chain;            // position in the chain of the Jacobi recurrences: host 0, then the others in index order
+    double eta_le, eta_lt;   // mass of the chain up to this body, with and without it
     double dt, m, I, R, sg, k2, kf, mh, Ih, M;
     double wk, wsat, wshape;   // this body's wind k factor, saturation and shape factor
     long long done;
@@ -106,6 +109,11 @@ __device__ __forceinline__ void put3(AdjLds &L, int row, int lane, V3 v) {
 }
 __device__ __forceinline__ V3 get3(const AdjLds &L, int row, int lane) {
     return V3{L.xb[row][lane], L.xb[row + 1][lane], L.xb[row + 2][lane]};
+}
+// Position of body j in the chain of the Jacobi recurrences (whfast.rs:889-933):
+// the host first, then the other bodies in index order.
+__device__ __forceinline__ int chain_of(const AdjLane &A, int j) {
+    return j == A.host ? 0 : (j < A.host ? j + 1 : j);
 }
 
 // One pass of the effects' vector-Jacobian product at the linearisation
@@ -379,6 +387,194 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     kick_adjoint(L, A, x0, v0, L0, v1, L1, h);
 }
 
+// ---- sums over the chain of the Jacobi recurrences (rows of 3 values) ----
+// sum over the bodies before this one in the chain of m_j * row_j
+__device__ __forceinline__ V3 chain_before_m(const AdjLds &L, const AdjLane &A, int row) {
+    V3 s{0., 0., 0.};
+    for (int j = 0; j < A.n; ++j) {
+        if (chain_of(A, j) >= A.chain) continue;
+        const int l = A.gbase + j;
+        s += L.m[l] * get3(L, row, l);
+    }
+    return s;
+}
+// sum over the bodies up to this one in the chain (itself included) of row_j
+__device__ __forceinline__ V3 chain_upto(const AdjLds &L, const AdjLane &A, int row) {
+    V3 s{0., 0., 0.};
+    for (int j = 0; j < A.n; ++j) {
+        if (chain_of(A, j) > A.chain) continue;
+        s += get3(L, row, A.gbase + j);
+    }
+    return s;
+}
+// sum over the non-host bodies from chain position c0 on of (m_j / eta_le_j) * row_j
+__device__ __forceinline__ V3 chain_from_w(const AdjLds &L, const AdjLane &A, int row, int c0) {
+    V3 s{0., 0., 0.};
+    for (int j = 0; j < A.n; ++j) {
+        if (j == A.host || chain_of(A, j) < c0) continue;
+        const int l = A.gbase + j;
+        s += (L.m[l] / L.eta[l]) * get3(L, row, l);
+    }
+    return s;
+}
+// sum over the non-host bodies after this one in the chain of row_j / eta_lt_j
+__device__ __forceinline__ V3 chain_after_inv(const AdjLds &L, const AdjLane &A, int row) {
+    V3 s{0., 0., 0.};
+    for (int j = 0; j < A.n; ++j) {
+        if (j == A.host || chain_of(A, j) <= A.chain) continue;
+        const int l = A.gbase + j;
+        s += (1. / (L.eta[l] - L.m[l])) * get3(L, row, l);
+    }
+    return s;
+}
+
+// Reverse of step `row` of the segment for a system in Jacobi coordinates
+// (whfast.rs:235-305 with the transforms of 889-963, 1034-1080: no jump; the
+// Kepler drifts take the chain's mass up to the body; the interaction adds
+// the Jacobi transform of the accelerations, which leave out the pair of the
+// host and the first body of the chain, and the indirect term of the bodies
+// after the first).  The transforms are linear in the positions and
+// velocities for fixed masses: with eta the chain's mass up to a body,
+//   ax_j = x_j - sum_{i before j} m_i x_i / eta_lt_j,   com = sum m_i x_i / M,
+//   x_j = ax_j + com - sum_{l from j on} (m_l / eta_le_l) ax_l,
+//   x_host = com - sum_{all l} (m_l / eta_le_l) ax_l,
+// and their vector-Jacobian products are the transposed sums.
+__device__ inline void adjoint_step_jacobi(const AdjArgs &Ar, AdjLds &L, AdjLane &A, int row) {
+    const TapeRef &T = Ar.T;
+    const int lane = A.lane;
+    const int b = A.body;
+    const V3 z{0., 0., 0.};
+    V3 x0 = z, v0 = z, L0 = z, v1 = z, L1 = z, x2 = z, v2 = z, L2 = z;
+    double X1 = 0., X2 = 0.;
+    if (A.valid) {
+        x0 = tape3(T, row, TP_X, b); v0 = tape3(T, row, TP_V, b); L0 = tape3(T, row, TP_L, b);
+        v1 = tape3(T, row, TP_V1, b); L1 = tape3(T, row, TP_L1, b);
+        X1 = *T.at(row, TP_XK1, b); X2 = *T.at(row, TP_XK2, b);
+        x2 = tape3(T, row + 1, TP_X, b); v2 = tape3(T, row + 1, TP_V, b); L2 = tape3(T, row + 1, TP_L, b);
+        A.R = *T.at(row, TP_R, b);
+        A.I = *T.at(row, TP_I, b);
+        L.R[lane] = A.R;
+        L.I[lane] = A.I;
+        if (A.wind) A.wshape = sqrt(A.R / R_SUN * 1. / A.m);
+    }
+    const double h = 0.5 * A.dt, dt = A.dt;
+    const double mu = posadj::G_K2 * A.eta_le;
+    const bool orb = A.orbiting();
+    const int hl = A.hl();
+    const int rank = A.chain - 1;        // 0 for the first body of the chain
+    const int first = A.host == 0 ? 1 : 0;   // index of the first body of the chain
+
+    // ---------------- recompute the drifts and the interaction ----------------
+    // to Jacobi coordinates from (x0, v1)
+    if (A.valid) { put3(L, 0, lane, x0); put3(L, 3, lane, v1); }
+    wsync();
+    A.Ih = L.I[hl];   // the host's moment of inertia of this step
+    const V3 com = (1. / A.M) * gsum3m(L, A, 0, false);
+    const V3 comv = (1. / A.M) * gsum3m(L, A, 3, false);
+    const V3 Q0 = orb ? x0 - (1. / A.eta_lt) * chain_before_m(L, A, 0) : z;
+    const V3 P0 = orb ? v1 - (1. / A.eta_lt) * chain_before_m(L, A, 3) : z;
+    wsync();
+    V3 Q1 = z, P1 = z;
+    if (orb) posadj::kepler_forward_x(mu, h, X1, Q0, P0, Q1, P1);
+    const V3 com1 = com + h * comv;
+    // inertial positions of the interaction
+    if (A.valid) put3(L, 0, lane, Q1);
+    wsync();
+    const V3 xg = Q1 + com1 - chain_from_w(L, A, 0, orb ? A.chain : 1);
+    wsync();
+    // gravitational accelerations (the host / first-body pair left out)
+    if (A.valid) put3(L, 0, lane, xg);
+    wsync();
+    V3 acc = z;
+    if (A.valid) {
+        for (int j = 0; j < A.n; ++j) {
+            if (j == A.k) continue;
+            if ((A.k == A.host && j == first) || (j == A.host && A.k == first)) continue;
+            const int l = A.gbase + j;
+            const V3 d = xg - get3(L, 0, l);
+            const double d2 = posadj::dot(d, d);
+            const double id = 1. / sqrt(d2);
+            acc += (-posadj::G_K2 * L.m[l] * id * id * id) * d;
+        }
+    }
+    wsync();
+    if (A.valid) put3(L, 0, lane, acc);
+    wsync();
+    const V3 aj = orb ? acc - (1. / A.eta_lt) * chain_before_m(L, A, 0) : z;
+    wsync();
+    // the indirect term of the bodies after the first (whfast.rs:955-962)
+    double r2 = 1., r3i = 0.;
+    if (orb && rank > 0) {
+        r2 = posadj::dot(Q1, Q1) + 1e-12;
+        r3i = 1. / (r2 * sqrt(r2));
+    }
+    const double gk = dt * posadj::G_K2 * A.eta_le;
+    const V3 P1pp = orb ? P1 + dt * aj + (rank > 0 ? gk * r3i : 0.) * Q1 : z;
+    V3 Q3 = z, P3 = z;
+    if (orb) posadj::kepler_forward_x(mu, h, X2, Q1, P1pp, Q3, P3);
+    // velocities after the second drift in inertial coordinates
+    if (A.valid) put3(L, 0, lane, P3);
+    wsync();
+    const V3 v2p = P3 + comv - chain_from_w(L, A, 0, orb ? A.chain : 1);
+    wsync();
+
+    // ---------------- reverse ----------------
+    // second kick (at x2, between (v2', L1) and (v2, L2))
+    kick_adjoint(L, A, x2, v2p, L1, v2, L2, h);
+    // back to inertial coordinates after the second drift
+    if (A.valid) { put3(L, 0, lane, ld3(A.lx)); put3(L, 3, lane, ld3(A.lv)); }
+    wsync();
+    const V3 Q3b = orb ? ld3(A.lx) - (A.m / A.eta_le) * chain_upto(L, A, 0) : z;
+    const V3 P3b = orb ? ld3(A.lv) - (A.m / A.eta_le) * chain_upto(L, A, 3) : z;
+    V3 com1b = gsum3(L, A, 0, false);   // com2 = com1 + h comv
+    V3 comvb = gsum3(L, A, 3, false) + h * com1b;
+    wsync();
+    // second Kepler drift
+    V3 Q1b = z, P1ppb = z;
+    if (orb) posadj::kepler_vjp(mu, h, X2, Q1, P1pp, Q3b, P3b, Q1b, P1ppb);
+    // interaction: the indirect term and the Jacobi accelerations
+    const V3 P1b = P1ppb;
+    if (orb && rank > 0) Q1b += (gk * r3i) * P1ppb - (3. * gk * r3i / r2 * posadj::dot(Q1, P1ppb)) * Q1;
+    if (A.valid) put3(L, 0, lane, orb ? dt * P1ppb : z);
+    wsync();
+    const V3 accb = (orb ? dt * P1ppb : z) - A.m * chain_after_inv(L, A, 0);
+    wsync();
+    // gravity
+    if (A.valid) { put3(L, 0, lane, accb); put3(L, 3, lane, xg); }
+    wsync();
+    V3 xgb = z;
+    if (A.valid) {
+        for (int j = 0; j < A.n; ++j) {
+            if (j == A.k) continue;
+            if ((A.k == A.host && j == first) || (j == A.host && A.k == first)) continue;
+            const int l = A.gbase + j;
+            const V3 d = xg - get3(L, 3, l);
+            xgb += posadj::gravity_pair_vjp(L.m[l], d, accb) - posadj::gravity_pair_vjp(A.m, d, get3(L, 0, l));
+        }
+    }
+    wsync();
+    // the interaction's inertial positions from (Q1, com1)
+    if (A.valid) put3(L, 0, lane, xgb);
+    wsync();
+    if (orb) Q1b += xgb - (A.m / A.eta_le) * chain_upto(L, A, 0);
+    com1b += gsum3(L, A, 0, false);
+    wsync();
+    // first Kepler drift (com1 = com + h comv)
+    V3 Q0b = z, P0b = z;
+    if (orb) posadj::kepler_vjp(mu, h, X1, Q0, P0, Q1b, P1b, Q0b, P0b);
+    comvb += h * com1b;
+    // to Jacobi coordinates from (x0, v1)
+    if (A.valid) { put3(L, 0, lane, Q0b); put3(L, 3, lane, P0b); }
+    wsync();
+    if (A.valid) {
+        st3(A.lx, Q0b - A.m * chain_after_inv(L, A, 0) + (A.m / A.M) * com1b);
+        st3(A.lv, P0b - A.m * chain_after_inv(L, A, 3) + (A.m / A.M) * comvb);
+    }
+    wsync();
+    // first kick (at x0, between (v0, L0) and (v1, L1))
+    kick_adjoint(L, A, x0, v0, L0, v1, L1, h);
+}
+
 __device__ inline void adjoint_load(const AdjArgs &Ar, AdjLds &L, AdjLane &A, int wave) {
     const DevEnsemble &E = Ar.E;
     const int lane = threadIdx.x;
@@ -438,6 +634,15 @@ __device__ inline void adjoint_load(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     double M = 0.;
     for (int j = 0; j < A.n; ++j) M += L.m[A.gbase + j];
     A.M = M;
+    A.jacobi = E.coord[s] == POS_COORD_JACOBI;
+    A.chain = chain_of(A, A.k);
+    double eta = 0.;
+    for (int j = 0; j < A.n; ++j)
+        if (chain_of(A, j) <= A.chain) eta += L.m[A.gbase + j];
+    A.eta_le = eta;
+    A.eta_lt = eta - A.m;
+    L.eta[lane] = eta;
+    __syncthreads();
 }
 
 __device__ inline void adjoint_store(const AdjArgs &Ar, const AdjLane &A) {

@@ -233,7 +233,8 @@ __global__ __launch_bounds__(64) void adjoint_segment_kernel(AdjArgs Ar) {
             // uniform within the group, not the wavefront: correct only because
             // the workgroup is this one wavefront (see whfast_steps_kernel)
             if (Ar.seg_start + j >= A.done) continue;
-            adjoint_step(Ar, L, A, j);
+            if (A.jacobi) adjoint_step_jacobi(Ar, L, A, j);
+            else adjoint_step(Ar, L, A, j);
         }
         adjoint_store(Ar, A);
         __syncthreads();
@@ -1695,8 +1696,8 @@ int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
     if (segment_steps < 0) return fail(POS_E_INVALID, "segment_steps must be >= 0");
     for (int ds = 0; ds < h->n_systems; ++ds) {
         const std::string where = "system " + std::to_string(h->sys_perm[ds]) + ": ";
-        if (h->coord_dev[ds] != POS_COORD_DEMOCRATIC_HELIOCENTRIC)
-            return fail(POS_E_UNSUPPORTED, where + "the adjoint supports democratic-heliocentric coordinates only");
+        if (h->coord_dev[ds] != POS_COORD_DEMOCRATIC_HELIOCENTRIC && h->coord_dev[ds] != POS_COORD_JACOBI)
+            return fail(POS_E_UNSUPPORTED, where + "the adjoint supports democratic-heliocentric and Jacobi coordinates only");
         if (h->eff_dev[ds] & POS_EFFECT_DISK)
             return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support the disk");
         if ((h->variant_dev[ds] & 3) != GRV_KIDDER)

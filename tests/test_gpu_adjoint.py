@@ -53,6 +53,27 @@ def per_system(spec, a):
     return np.add.reduceat(a, spec.body_offset[:-1])
 
 
+def check_against_tangent(ens, spec, w_rng, d, tan, label, tol=1e-6):
+    """Every seed block against every tangent direction: the adjoint's
+    g . d must equal w . (J d) to `tol` relative, system by system."""
+    fin = ens.get_state()
+    for blk in BLOCKS:
+        w = w_rng.standard_normal((spec.n_bodies, 3)) / np.abs(fin[blk]).max(axis=1, keepdims=True)
+        g = ens.adjoint(**{blk: w})
+        tkey = {"position": "x", "velocity": "v", "angular_momentum": "L"}[blk]
+        for k in range(8):
+            lhs = per_system(spec, w * tan[tkey][k])
+            rhs = (per_system(spec, g["position"] * d["x"][k]) + per_system(spec, g["velocity"] * d["v"][k])
+                   + per_system(spec, g["angular_momentum"] * d["L"][k])
+                   + per_system(spec, g["dissipation_factor"] * d["sigma"][k])
+                   + per_system(spec, g["love_number"] * d["k2"][k])
+                   + per_system(spec, g["fluid_love_number"] * d["k2f"][k]))
+            scale = np.maximum(np.abs(lhs), np.abs(rhs))
+            rel = np.abs(lhs - rhs) / np.where(scale > 0, scale, 1.)
+            print(f"{label}: seed {blk:17s} dir {k}: max rel {rel.max():.2e}", flush=True)
+            assert rel.max() < tol, (blk, k, rel.max(), lhs[:3], rhs[:3])
+
+
 @pytest.mark.parametrize("steps,segment", [(200, 64), (37, 1000), (10_000, 1000), (100_000, 1000)])
 def test_adjoint_equals_tangent_linear(Ensemble, oracle, steps, segment):
     """16 systems of the adjoint workload, up to its full horizon (10^5
@@ -141,21 +162,7 @@ def test_adjoint_with_wind_equals_tangent_linear(Ensemble, oracle):
         fin = ens.get_state()
         for k in BLOCKS:
             np.testing.assert_array_equal(fin[k], ref[k], err_msg=f"recorded forward run differs in {k}")
-        for blk in BLOCKS:
-            w = rng.standard_normal((spec.n_bodies, 3)) / np.abs(fin[blk]).max(axis=1, keepdims=True)
-            g = ens.adjoint(**{blk: w})
-            tkey = {"position": "x", "velocity": "v", "angular_momentum": "L"}[blk]
-            for k in range(8):
-                lhs = per_system(spec, w * tan[tkey][k])
-                rhs = (per_system(spec, g["position"] * d["x"][k]) + per_system(spec, g["velocity"] * d["v"][k])
-                       + per_system(spec, g["angular_momentum"] * d["L"][k])
-                       + per_system(spec, g["dissipation_factor"] * d["sigma"][k])
-                       + per_system(spec, g["love_number"] * d["k2"][k])
-                       + per_system(spec, g["fluid_love_number"] * d["k2f"][k]))
-                scale = np.maximum(np.abs(lhs), np.abs(rhs))
-                rel = np.abs(lhs - rhs) / np.where(scale > 0, scale, 1.)
-                print(f"wind: seed {blk:17s} dir {k}: max rel {rel.max():.2e}", flush=True)
-                assert rel.max() < 1e-6, (blk, k, rel.max(), lhs[:3], rhs[:3])
+        check_against_tangent(ens, spec, rng, d, tan, "wind")
 
 
 def evolving_workload(n_systems, steps):
@@ -214,21 +221,44 @@ def test_adjoint_with_evolution_equals_tangent_linear(Ensemble, oracle):
         fin = ens.get_state()
         for k in BLOCKS + ("radius", "moment_of_inertia"):
             np.testing.assert_array_equal(fin[k], ref[k], err_msg=f"recorded forward run differs in {k}")
-        for blk in BLOCKS:
-            w = rng.standard_normal((spec.n_bodies, 3)) / np.abs(fin[blk]).max(axis=1, keepdims=True)
-            g = ens.adjoint(**{blk: w})
-            tkey = {"position": "x", "velocity": "v", "angular_momentum": "L"}[blk]
-            for k in range(8):
-                lhs = per_system(spec, w * tan[tkey][k])
-                rhs = (per_system(spec, g["position"] * d["x"][k]) + per_system(spec, g["velocity"] * d["v"][k])
-                       + per_system(spec, g["angular_momentum"] * d["L"][k])
-                       + per_system(spec, g["dissipation_factor"] * d["sigma"][k])
-                       + per_system(spec, g["love_number"] * d["k2"][k])
-                       + per_system(spec, g["fluid_love_number"] * d["k2f"][k]))
-                scale = np.maximum(np.abs(lhs), np.abs(rhs))
-                rel = np.abs(lhs - rhs) / np.where(scale > 0, scale, 1.)
-                print(f"evolution: seed {blk:17s} dir {k}: max rel {rel.max():.2e}", flush=True)
-                assert rel.max() < 1e-6, (blk, k, rel.max(), lhs[:3], rhs[:3])
+        check_against_tangent(ens, spec, rng, d, tan, "evolution")
+
+
+def jacobi_workloads():
+    """Systems in Jacobi coordinates (whfast.rs:889-963, 1034-1080): the
+    star + 3 planet workload switched to Jacobi coordinates, the binary host
+    with 1-3 circumbinary planets (3-5 bodies: the host / first body pair
+    left out of the gravity, the indirect term on the planets), and the two
+    mixed with democratic-heliocentric systems so that lanes of both kinds
+    of reverse step share wavefronts."""
+    from posidonius_amd import abi
+    from posidonius_amd.case import EnsembleSpec
+    four = gen.config4(16)
+    four.system["coordinates"][:] = abi.COORD_JACOBI
+    mixed = EnsembleSpec.concat([gen.config4(8, seed=31), gen.config3(8, seed=33), gen.config2(8, seed=35)])
+    return {"star_3planets": four, "binary_circumbinary": gen.config3(16), "mixed": mixed}
+
+
+@pytest.mark.parametrize("name", ["star_3planets", "binary_circumbinary", "mixed"])
+@pytest.mark.parametrize("steps,segment", [(300, 100), (3_000, 1000)])
+def test_adjoint_in_jacobi_coordinates_equals_tangent_linear(Ensemble, oracle, name, steps, segment):
+    """Jacobi coordinates: the reverse step's transforms, Kepler drifts of the
+    chain's masses, skipped pair and indirect term against the tangent-linear
+    run, every seed block and direction to 1e-6 relative."""
+    from posidonius_amd import abi
+    spec = jacobi_workloads()[name]
+    n_jacobi = int(np.sum(spec.system["coordinates"] == abi.COORD_JACOBI))
+    print(f"adjoint in Jacobi coordinates: {name}, {spec.n_systems} systems ({n_jacobi} Jacobi) x {steps} steps, "
+          f"segments of {segment}", flush=True)
+    rng = np.random.default_rng(41)
+    d = directions(spec, rng)
+    ref, tan = oracle.tangent(spec, steps, d, threads=max(1, min(16, os.cpu_count() or 1)))
+    with Ensemble(spec, mapping="packed", steps_per_launch=64) as ens:
+        ens.adjoint_record(steps, segment_steps=segment)
+        fin = ens.get_state()
+        for k in BLOCKS:
+            np.testing.assert_array_equal(fin[k], ref[k], err_msg=f"recorded forward run differs in {k}")
+        check_against_tangent(ens, spec, rng, d, tan, f"jacobi {name}")
 
 
 def test_adjoint_is_the_same_for_every_launch_mapping(Ensemble):
