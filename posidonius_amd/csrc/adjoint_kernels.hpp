@@ -68,6 +68,7 @@ struct AdjArgs {
 struct AdjLane {
     int lane, gbase, k, n, host, sys, body;
     bool valid, is_host, tides, flat, gr, spin, wind, jacobi;
+    bool evolving;        // evolution tables: radius and moment of inertia follow time
     int chain;            // position in the chain of the Jacobi recurrences: host 0, then the others in index order
     double eta_le, eta_lt;   // mass of the chain up to this body, with and without it
     double dt, m, I, R, sg, k2, kf, mh, Ih, M;
@@ -244,6 +245,19 @@ __device__ __forceinline__ V3 tape3(const TapeRef &T, int row, int field, int bo
     return V3{*T.at(row, field, body), *T.at(row, field + 1, body), *T.at(row, field + 2, body)};
 }
 
+// This step's radius and moment of inertia.  Under evolution tables they
+// follow time and depend on no state, so they carry no derivative and come
+// from the tape; without them they keep the values loaded at the segment's
+// start.  The host's moment of inertia is read back after the next wsync.
+__device__ __forceinline__ void step_radii(const TapeRef &T, AdjLds &L, AdjLane &A, int row) {
+    if (!(A.valid && A.evolving)) return;
+    A.R = *T.at(row, TP_R, A.body);
+    A.I = *T.at(row, TP_I, A.body);
+    L.R[A.lane] = A.R;
+    L.I[A.lane] = A.I;
+    if (A.wind) A.wshape = sqrt(A.R / R_SUN * 1. / A.m);
+}
+
 // Reverse of step `row` of the segment: A.l* hold the cotangents of the
 // state after the step on entry and before it on exit.
 __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, int row) {
@@ -258,14 +272,8 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
         v1 = tape3(T, row, TP_V1, b); L1 = tape3(T, row, TP_L1, b);
         X1 = *T.at(row, TP_XK1, b); X2 = *T.at(row, TP_XK2, b);
         x2 = tape3(T, row + 1, TP_X, b); v2 = tape3(T, row + 1, TP_V, b); L2 = tape3(T, row + 1, TP_L, b);
-        // this step's radius and moment of inertia (they follow the evolution
-        // tables in time and depend on no state, so they carry no derivative)
-        A.R = *T.at(row, TP_R, b);
-        A.I = *T.at(row, TP_I, b);
-        L.R[lane] = A.R;
-        L.I[lane] = A.I;
-        if (A.wind) A.wshape = sqrt(A.R / R_SUN * 1. / A.m);
     }
+    step_radii(T, L, A, row);
     const double h = 0.5 * A.dt, dt = A.dt;
     const double mu = posadj::G_K2 * A.mh;
     const bool orb = A.orbiting();
@@ -274,7 +282,7 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     // ---------------- recompute the drift from (x0, v1) ----------------
     if (A.valid) { put3(L, 0, lane, x0); put3(L, 3, lane, v1); }
     wsync();
-    A.Ih = L.I[hl];   // the host's moment of inertia of this step
+    if (A.evolving) A.Ih = L.I[hl];   // the host's moment of inertia of this step
     const V3 Xc = (1. / A.M) * gsum3m(L, A, 0, false);
     const V3 V = (1. / A.M) * gsum3m(L, A, 3, false);
     const V3 Q0 = orb ? x0 - get3(L, 0, hl) : z;
@@ -451,12 +459,8 @@ __device__ inline void adjoint_step_jacobi(const AdjArgs &Ar, AdjLds &L, AdjLane
         v1 = tape3(T, row, TP_V1, b); L1 = tape3(T, row, TP_L1, b);
         X1 = *T.at(row, TP_XK1, b); X2 = *T.at(row, TP_XK2, b);
         x2 = tape3(T, row + 1, TP_X, b); v2 = tape3(T, row + 1, TP_V, b); L2 = tape3(T, row + 1, TP_L, b);
-        A.R = *T.at(row, TP_R, b);
-        A.I = *T.at(row, TP_I, b);
-        L.R[lane] = A.R;
-        L.I[lane] = A.I;
-        if (A.wind) A.wshape = sqrt(A.R / R_SUN * 1. / A.m);
     }
+    step_radii(T, L, A, row);
     const double h = 0.5 * A.dt, dt = A.dt;
     const double mu = posadj::G_K2 * A.eta_le;
     const bool orb = A.orbiting();
@@ -468,7 +472,7 @@ __device__ inline void adjoint_step_jacobi(const AdjArgs &Ar, AdjLds &L, AdjLane
     // to Jacobi coordinates from (x0, v1)
     if (A.valid) { put3(L, 0, lane, x0); put3(L, 3, lane, v1); }
     wsync();
-    A.Ih = L.I[hl];   // the host's moment of inertia of this step
+    if (A.evolving) A.Ih = L.I[hl];   // the host's moment of inertia of this step
     const V3 com = (1. / A.M) * gsum3m(L, A, 0, false);
     const V3 comv = (1. / A.M) * gsum3m(L, A, 3, false);
     const V3 Q0 = orb ? x0 - (1. / A.eta_lt) * chain_before_m(L, A, 0) : z;
@@ -596,6 +600,7 @@ __device__ inline void adjoint_load(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     A.flat = eff & POS_EFFECT_ROTATIONAL_FLATTENING;
     A.gr = eff & POS_EFFECT_GENERAL_RELATIVITY;
     A.spin = eff & (POS_EFFECT_TIDES | POS_EFFECT_ROTATIONAL_FLATTENING | POS_EFFECT_GENERAL_RELATIVITY | POS_EFFECT_EVOLUTION);
+    A.evolving = eff & POS_EFFECT_EVOLUTION;
     // (the recording run admits no GR variant other than Kidder 1995)
     A.dt = E.dt[s];
     A.done = Ar.steps_done[s];
