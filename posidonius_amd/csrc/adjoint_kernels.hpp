@@ -249,8 +249,11 @@ __device__ __forceinline__ V3 tape3(const TapeRef &T, int row, int field, int bo
 // follow time and depend on no state, so they carry no derivative and come
 // from the tape; without them they keep the values loaded at the segment's
 // start.  The host's moment of inertia is read back after the next wsync.
+// EVO: the launch holds systems with evolution tables (without them the
+// reads are compiled out).
+template <bool EVO>
 __device__ __forceinline__ void step_radii(const TapeRef &T, AdjLds &L, AdjLane &A, int row) {
-    if (!(A.valid && A.evolving)) return;
+    if (!(EVO && A.valid && A.evolving)) return;
     A.R = *T.at(row, TP_R, A.body);
     A.I = *T.at(row, TP_I, A.body);
     L.R[A.lane] = A.R;
@@ -260,6 +263,7 @@ __device__ __forceinline__ void step_radii(const TapeRef &T, AdjLds &L, AdjLane 
 
 // Reverse of step `row` of the segment: A.l* hold the cotangents of the
 // state after the step on entry and before it on exit.
+template <bool EVO>
 __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, int row) {
     const TapeRef &T = Ar.T;
     const int lane = A.lane;
@@ -273,7 +277,7 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
         X1 = *T.at(row, TP_XK1, b); X2 = *T.at(row, TP_XK2, b);
         x2 = tape3(T, row + 1, TP_X, b); v2 = tape3(T, row + 1, TP_V, b); L2 = tape3(T, row + 1, TP_L, b);
     }
-    step_radii(T, L, A, row);
+    step_radii<EVO>(T, L, A, row);
     const double h = 0.5 * A.dt, dt = A.dt;
     const double mu = posadj::G_K2 * A.mh;
     const bool orb = A.orbiting();
@@ -282,7 +286,7 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     // ---------------- recompute the drift from (x0, v1) ----------------
     if (A.valid) { put3(L, 0, lane, x0); put3(L, 3, lane, v1); }
     wsync();
-    if (A.evolving) A.Ih = L.I[hl];   // the host's moment of inertia of this step
+    if (EVO && A.evolving) A.Ih = L.I[hl];   // the host's moment of inertia of this step
     const V3 Xc = (1. / A.M) * gsum3m(L, A, 0, false);
     const V3 V = (1. / A.M) * gsum3m(L, A, 3, false);
     const V3 Q0 = orb ? x0 - get3(L, 0, hl) : z;
@@ -447,6 +451,7 @@ __device__ __forceinline__ V3 chain_after_inv(const AdjLds &L, const AdjLane &A,
 //   x_j = ax_j + com - sum_{l from j on} (m_l / eta_le_l) ax_l,
 //   x_host = com - sum_{all l} (m_l / eta_le_l) ax_l,
 // and their vector-Jacobian products are the transposed sums.
+template <bool EVO>
 __device__ inline void adjoint_step_jacobi(const AdjArgs &Ar, AdjLds &L, AdjLane &A, int row) {
     const TapeRef &T = Ar.T;
     const int lane = A.lane;
@@ -460,7 +465,7 @@ __device__ inline void adjoint_step_jacobi(const AdjArgs &Ar, AdjLds &L, AdjLane
         X1 = *T.at(row, TP_XK1, b); X2 = *T.at(row, TP_XK2, b);
         x2 = tape3(T, row + 1, TP_X, b); v2 = tape3(T, row + 1, TP_V, b); L2 = tape3(T, row + 1, TP_L, b);
     }
-    step_radii(T, L, A, row);
+    step_radii<EVO>(T, L, A, row);
     const double h = 0.5 * A.dt, dt = A.dt;
     const double mu = posadj::G_K2 * A.eta_le;
     const bool orb = A.orbiting();
@@ -472,7 +477,7 @@ __device__ inline void adjoint_step_jacobi(const AdjArgs &Ar, AdjLds &L, AdjLane
     // to Jacobi coordinates from (x0, v1)
     if (A.valid) { put3(L, 0, lane, x0); put3(L, 3, lane, v1); }
     wsync();
-    if (A.evolving) A.Ih = L.I[hl];   // the host's moment of inertia of this step
+    if (EVO && A.evolving) A.Ih = L.I[hl];   // the host's moment of inertia of this step
     const V3 com = (1. / A.M) * gsum3m(L, A, 0, false);
     const V3 comv = (1. / A.M) * gsum3m(L, A, 3, false);
     const V3 Q0 = orb ? x0 - (1. / A.eta_lt) * chain_before_m(L, A, 0) : z;

@@ -224,10 +224,11 @@ void whfast_record_kernel(DevEnsemble E, int n_steps, TapeRef T, int wave_begin,
 }
 
 // Reverse sweep over one recorded segment, last step first.  JACOBI: the
-// ensemble holds systems in Jacobi coordinates (without them the Jacobi
-// reverse step is compiled out, which keeps the heliocentric sweep's
-// register allocation as it is alone).
-template <bool JACOBI>
+// ensemble holds systems in Jacobi coordinates; EVO: systems with evolution
+// tables.  Without them the Jacobi reverse step and the per-step tape reads
+// of the radii are compiled out, which keeps the heliocentric sweep's
+// register allocation as it is alone.
+template <bool JACOBI, bool EVO>
 __global__ __launch_bounds__(64) void adjoint_segment_kernel(AdjArgs Ar) {
     __shared__ AdjLds L;
     for (int wave = blockIdx.x; wave < Ar.E.n_waves; wave += gridDim.x) {
@@ -237,8 +238,8 @@ __global__ __launch_bounds__(64) void adjoint_segment_kernel(AdjArgs Ar) {
             // uniform within the group, not the wavefront: correct only because
             // the workgroup is this one wavefront (see whfast_steps_kernel)
             if (Ar.seg_start + j >= A.done) continue;
-            if (JACOBI && A.jacobi) adjoint_step_jacobi(Ar, L, A, j);
-            else adjoint_step(Ar, L, A, j);
+            if (JACOBI && A.jacobi) adjoint_step_jacobi<EVO>(Ar, L, A, j);
+            else adjoint_step<EVO>(Ar, L, A, j);
         }
         adjoint_store(Ar, A);
         __syncthreads();
@@ -1839,6 +1840,10 @@ int32_t pos_adjoint(pos_ens *h, const pos_adjoint_seed *seed, pos_adjoint_grad *
         return POS_OK;
     };
     const bool jacobi = std::find(h->coord_dev.begin(), h->coord_dev.end(), POS_COORD_JACOBI) != h->coord_dev.end();
+    const bool evo = std::any_of(h->eff_dev.begin(), h->eff_dev.end(),
+                                 [](uint32_t e) { return (e & POS_EFFECT_EVOLUTION) != 0; });
+    void (*sweep_kernel)(AdjArgs) = jacobi ? (evo ? adjoint_segment_kernel<true, true> : adjoint_segment_kernel<true, false>)
+                                    : (evo ? adjoint_segment_kernel<false, true> : adjoint_segment_kernel<false, false>);
     // the sweep stream must also see the seeds uploaded on the handle's stream
     HIP_TRY(hipEventRecord(A.swept_ev[0], h->stream));
     HIP_TRY(hipStreamWaitEvent(A.sweep, A.swept_ev[0], 0));
@@ -1849,10 +1854,8 @@ int32_t pos_adjoint(pos_ens *h, const pos_adjoint_seed *seed, pos_adjoint_grad *
         Ar.T = TapeRef{A.tape[buf], nb};
         Ar.seg_start = (int64_t)s * A.seg;
         Ar.seg_len = seg_len(s);
-        if (jacobi)
-            hipLaunchKernelGGL(adjoint_segment_kernel<true>, dim3(h->grid), dim3(64), 0, A.sweep, Ar);
-        else
-            hipLaunchKernelGGL(adjoint_segment_kernel<false>, dim3(h->grid), dim3(64), 0, A.sweep, Ar);
+        void *args[] = {&Ar};
+        HIP_TRY(hipLaunchKernel((const void *)sweep_kernel, dim3(h->grid), dim3(64), args, 0, A.sweep));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(A.swept_ev[buf], A.sweep));
         if (s > 0 && (rc = recompute(s - 1))) return rc;
