@@ -236,10 +236,30 @@ def jacobi_workloads():
     four = gen.config4(16)
     four.system["coordinates"][:] = abi.COORD_JACOBI
     mixed = EnsembleSpec.concat([gen.config4(8, seed=31), gen.config3(8, seed=33), gen.config2(8, seed=35)])
-    return {"star_3planets": four, "binary_circumbinary": gen.config3(16), "mixed": mixed}
+    return {"star_3planets": four, "binary_circumbinary": gen.config3(16), "mixed": mixed,
+            "host_moved": host_moved(gen.config3(16, seed=37), np.random.default_rng(37))}
 
 
-@pytest.mark.parametrize("name", ["star_3planets", "binary_circumbinary", "mixed"])
+def host_moved(spec, rng):
+    """The same systems with each host (body 0) moved to a random index of its
+    system, the other bodies keeping their order: the chain of the Jacobi
+    recurrences then runs host first and the others by index, and the pair
+    left out of the gravity is the host and the lowest other index."""
+    out = spec.copy()
+    perm = np.arange(spec.n_bodies)
+    for s in range(spec.n_systems):
+        b0, b1 = int(spec.body_offset[s]), int(spec.body_offset[s + 1])
+        local = list(range(1, b1 - b0))
+        local.insert(int(rng.integers(1, b1 - b0)), 0)
+        perm[b0:b1] = b0 + np.asarray(local)
+    for k, v in out.body.items():
+        if v is not None:
+            out.body[k] = np.ascontiguousarray(v[perm])
+    out.elements = {k: np.ascontiguousarray(v[perm]) for k, v in spec.elements.items()}
+    return out
+
+
+@pytest.mark.parametrize("name", ["star_3planets", "binary_circumbinary", "mixed", "host_moved"])
 @pytest.mark.parametrize("steps,segment", [(300, 100), (3_000, 1000)])
 def test_adjoint_in_jacobi_coordinates_equals_tangent_linear(Ensemble, oracle, name, steps, segment):
     """Jacobi coordinates: the reverse step's transforms, Kepler drifts of the
