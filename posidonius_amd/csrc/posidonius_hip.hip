@@ -138,7 +138,7 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
         lds.s[q][lane] = sp[q];
         lds.ag[q][lane] = E.acc[q * nb + b];
     }
-    lds.s2[lane] = p2(sp[0]) + p2(sp[1]) + p2(sp[2]);
+    if (lane < NX) lds.xb[lane][WAVE] = 0.;   // the exchange area's zero column
 #pragma unroll
     for (int q = 0; q < 3; ++q) { B.ax[q] = 0.; B.av[q] = 0.; B.comx[q] = 0.; B.comv[q] = 0.; }
     B.com_m = 0.;

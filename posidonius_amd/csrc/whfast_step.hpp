@@ -156,10 +156,13 @@ struct WaveLds {
     // 1 / (m_h + m) (general_relativity.rs:326)
     double kf[WAVE], inv_M[WAVE];
     double R[WAVE], I[WAVE];
-    double s[3][WAVE], s2[WAVE];   // spin of the last evaluation and its squared norm
+    double s[3][WAVE];             // spin of the last evaluation
     double ag[3][WAVE];            // gravitational acceleration (universe.rs:300-302)
-    // phase-dependent exchange
-    double xb[NX][WAVE];
+    // phase-dependent exchange; column WAVE holds +0 in every row (written
+    // once per wavefront, never by a lane): the ordered folds read it for
+    // the slots they skip.  The odd row stride also spreads the folds' reads
+    // of one column in several rows over the LDS banks.
+    double xb[NX][WAVE + 1];
 };
 
 // Every kernel that uses the exchange area runs one wavefront per workgroup
@@ -342,7 +345,12 @@ __device__ __forceinline__ void own_spin(WaveLds &lds, Lane &B) {
     const Rcp RI = rcp(I);
     const double s0 = divr(B.L[0], RI), s1 = divr(B.L[1], RI), s2 = divr(B.L[2], RI);
     lds.s[0][B.lane] = s0; lds.s[1][B.lane] = s1; lds.s[2][B.lane] = s2;
-    lds.s2[B.lane] = p2(s0) + p2(s1) + p2(s2);
+}
+
+// |spin|^2 of lane l's last evaluation (the same rounded value as when the
+// spin is formed: the same products summed in the same order)
+__device__ __forceinline__ double spin2(const WaveLds &lds, int l) {
+    return p2(lds.s[0][l]) + p2(lds.s[1][l]) + p2(lds.s[2][l]);
 }
 
 // evolution.rs:548-567 for this lane's body (after its spin is refreshed):
@@ -355,7 +363,7 @@ __device__ inline void evolve_spin_dependent(const DevEnsemble &E, WaveLds &lds,
     int left = 0;
     const double itq = interp_table(E.tab_time + off + i0, E.tab_itq + off + i0, len - i0, t, &left);
     B.left_index += left;
-    const double epsilon_squared = lds.s2[B.lane] / SUN_DYN_FREQ_2;
+    const double epsilon_squared = spin2(lds, B.lane) / SUN_DYN_FREQ_2;
     E.lag[B.body] = 3.0 * epsilon_squared * itq / 4.0;
 }
 
@@ -382,7 +390,8 @@ __device__ __forceinline__ void put3(WaveLds &lds, int slot, int lane, double x,
 // terms of FOLD_BATCH bodies of every row are formed first (their LDS loads
 // issue together and one wait covers them), then folded row by row in body
 // order.  Slots past the group's last body, and the host's slot when
-// SKIP_HOST, fold +0.  Every reduction folded this way starts at +0 and adds
+// SKIP_HOST, fold +0: they read the exchange area's zero column WAVE (the
+// load callback indexes the exchange rows).  Every reduction folded this way starts at +0 and adds
 // its terms, so it is never -0 (in round-to-nearest x + y is -0 only when
 // both x and y are); adding +0 to a value that is not -0 leaves it unchanged
 // bit for bit, so each result is the sequential sum over the bodies it names.
@@ -399,12 +408,9 @@ __device__ __forceinline__ void fold_rows(const Lane &B, double (&acc)[R], Load 
         for (int u = 0; u < FOLD_BATCH; ++u) {
             const int j = j0 + u;
             const bool use = j < B.n && !(SKIP_HOST && j == B.host);
-            const int l = B.gbase + (use ? j : 0);   // body 0 keeps every index in the group
+            const int l = use ? B.gbase + j : WAVE;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const double v = load(r, l);
-                t[r][u] = use ? v : 0.;
-            }
+            for (int r = 0; r < R; ++r) t[r][u] = load(r, l);
         }
 #pragma unroll
         for (int u = 0; u < FOLD_BATCH; ++u)
@@ -981,7 +987,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
         const double m = B.m, hm = lds.m[hl];
         const double hs[3] = {lds.s[0][hl], lds.s[1][hl], lds.s[2][hl]};
         const double s[3] = {lds.s[0][lane], lds.s[1][lane], lds.s[2][lane]};
-        const double hs2 = lds.s2[hl], s2 = lds.s2[lane];
+        const double hs2 = p2(hs[0]) + p2(hs[1]) + p2(hs[2]), s2 = p2(s[0]) + p2(s[1]) + p2(s[2]);
         const double hR = lds.R[hl], R = lds.R[lane];
         const double d7 = H ? 0. : p7(d);
         const Rcp Rd7 = H ? Rcp{0., 0.} : rcp(d7);
@@ -1235,8 +1241,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
     if (B.valid()) host_reaction_sums(lds, B);
     wsync();
     if (B.valid() && B.is_host()) {
-        const double *col = &lds.xb[0][lane];   // this lane's column, row stride WAVE
-        auto row = [&](int q) { return col[q * WAVE]; };
+        auto row = [&](int q) { return lds.xb[q][lane]; };   // this lane's column
         if (LT.tides) {   // tides/common.rs:312-345 and 223-261
 #pragma unroll
             for (int q = 0; q < 3; ++q) a_add[q] = a_add[q] + -1.0 * B.inv_m * row(C_TF + q);
@@ -1287,7 +1292,7 @@ __device__ inline void additional_effects(const DevEnsemble &E, WaveLds &lds, La
     double w[3] = {0., 0., 0.};
     if (B.valid() && (B.flags & F_WIND) && LT.spin && (B.flags & F_WIND_ROLE)) {
         const double wk = E.wind_k[B.body], wsat = E.wind_sat[B.body];
-        const double s2 = lds.s2[lane];
+        const double s2 = spin2(lds, lane);
         const double threshold = sqrt(s2);
         const double shape = sqrt(lds.R[lane] / R_SUN * 1. / B.m);
         const double f = threshold >= wsat ? wsat * wsat : s2;
