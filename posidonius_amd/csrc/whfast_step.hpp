@@ -398,8 +398,8 @@ __device__ __forceinline__ void put3(WaveLds &lds, int slot, int lane, double x,
 // FOLD_BATCH and the rows per pass of the reaction sums (REACTION_ROWS) are
 // the fastest of the settings measured on the benchmark ensemble: the loads in
 // flight cost registers.
-constexpr int FOLD_BATCH = 3;
-constexpr int REACTION_ROWS = 2;
+constexpr int FOLD_BATCH = 4;
+constexpr int REACTION_ROWS = 3;
 template <int R, bool SKIP_HOST, class Load>
 __device__ __forceinline__ void fold_rows(const Lane &B, double (&acc)[R], Load load) {
     for (int j0 = 0; j0 < B.n; j0 += FOLD_BATCH) {
