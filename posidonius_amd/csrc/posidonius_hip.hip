@@ -139,6 +139,7 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
         lds.ag[q][lane] = E.acc[q * nb + b];
     }
     if (lane < NX) lds.xb[lane][WAVE] = 0.;   // the exchange area's zero column
+    if (lane == 0) lds.m[WAVE] = 0.;
 #pragma unroll
     for (int q = 0; q < 3; ++q) { B.ax[q] = 0.; B.av[q] = 0.; B.comx[q] = 0.; B.comv[q] = 0.; }
     B.com_m = 0.;

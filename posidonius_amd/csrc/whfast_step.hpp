@@ -150,7 +150,8 @@ __host__ __device__ __forceinline__ bool evo_dynamical(int type) {
 constexpr int NX = 24;   // exchange slots per lane
 struct WaveLds {
     // persistent per body (lane)
-    double m[WAVE], inv_m[WAVE], sigma[WAVE], k2[WAVE], k2f[WAVE];
+    double m[WAVE + 1];   // entry WAVE is +0, the mass of the exchange area's zero column
+    double inv_m[WAVE], sigma[WAVE], k2[WAVE], k2f[WAVE];
     // of an orbiting body's pair with its host (masses only): the Kidder mass
     // factor f = G m_h G m / (G m_h + G m)^2 (general_relativity.rs:91) and
     // 1 / (m_h + m) (general_relativity.rs:326)
@@ -1600,17 +1601,16 @@ __device__ inline void inertial_to_alternative(WaveLds &lds, Lane &B) {
         for (int q = 0; q < 3; ++q) { B.comx[q] = s[q] * mtot_i; B.comv[q] = sv[q] * mtot_i; }
     } else {
         // whfast.rs:973-1023: centre of mass, host first then the others in
-        // order; the seven ordered sums are spread over the group's lanes and
-        // left in rows 6-12 of column gbase
+        // order; the seven ordered sums (x, v weighted by the masses, and the
+        // masses themselves as 1 * m from row 13) are spread over the group's
+        // lanes and left in rows 6-12 of column gbase
+        if (B.valid()) lds.xb[13][lane] = 1.;
+        wsync();
         for (int q = B.k; B.valid() && q < 7; q += B.n) {
-            double acc = 0.;
-            for (int jj = -1; jj < B.n; ++jj) {
-                const int j = jj < 0 ? B.host : jj;
-                if (jj >= 0 && jj == B.host) continue;
-                const int l = B.gbase + j;
-                acc = acc + (q < 6 ? lds.xb[q][l] * lds.m[l] : lds.m[l]);
-            }
-            lds.xb[6 + q][B.gbase] = acc;
+            const int rq = q < 6 ? q : 13;
+            double acc[1] = {0. + lds.xb[rq][hl] * lds.m[hl]};
+            fold_rows<1, true>(B, acc, [&](int, int l) { return lds.xb[rq][l] * lds.m[l]; });
+            lds.xb[6 + q][B.gbase] = acc[0];
         }
         wsync();
         const double cx[3] = {lds.xb[6][B.gbase], lds.xb[7][B.gbase], lds.xb[8][B.gbase]};
@@ -1650,15 +1650,15 @@ __device__ inline void jump_step(WaveLds &lds, Lane &B, double dt) {
     // three ordered momentum sums, spread over the lanes (valid lanes only:
     // an empty group has n = 0), left in the host's column
     for (int q = B.k; B.valid() && q < 3; q += B.n) {
-        double acc = 0.;
-        for (int j = 0; j < B.n; ++j) {
-            if (j == B.host) continue;
-            const int l = B.gbase + j;
-            const double mj = lds.m[l];
-            if (B.coord == POS_COORD_DEMOCRATIC_HELIOCENTRIC) acc = acc + mj * lds.xb[q][l];
-            else acc = acc + divr(mj * lds.xb[q][l], rcp(m0 + mj));
-        }
-        lds.xb[q][hl] = acc;
+        double acc[1] = {0.};
+        if (B.coord == POS_COORD_DEMOCRATIC_HELIOCENTRIC)
+            fold_rows<1, true>(B, acc, [&](int, int l) { return lds.m[l] * lds.xb[q][l]; });
+        else
+            fold_rows<1, true>(B, acc, [&](int, int l) {
+                const double mj = lds.m[l];
+                return divr(mj * lds.xb[q][l], rcp(m0 + mj));
+            });
+        lds.xb[q][hl] = acc[0];
     }
     wsync();
     const double p[3] = {lds.xb[0][hl], lds.xb[1][hl], lds.xb[2][hl]};
@@ -1780,22 +1780,20 @@ __device__ inline void alternative_to_inertial(WaveLds &lds, Lane &B) {
         const Rcp Rm0{m0, lds.inv_m[hl]};
         // six ordered chains (host position and velocity), one per lane,
         // left in the host's column
+        // (acc - t is acc + (-t) bit for bit; a skipped slot's term is -0,
+        // which leaves every value, -0 included, unchanged)
         for (int q = B.k; B.valid() && q < 6; q += B.n) {
             // constant indices only: a dynamic index would move comx/comv to the stack
-            double acc = q == 0 ? B.comx[0] : q == 1 ? B.comx[1] : q == 2 ? B.comx[2]
-                       : q == 3 ? B.comv[0] : q == 4 ? B.comv[1] : B.comv[2];
-            for (int j = 0; j < B.n; ++j) {
-                if (j == B.host) continue;
-                const int l = B.gbase + j;
-                const double mj = lds.m[l];
-                if (q < 3) {
-                    acc = acc - divr(lds.xb[q][l] * mj, Rmt);
-                } else {
-                    const double factor = (B.coord == POS_COORD_WHDS) ? mj / (m0 + mj) : divr(mj, Rm0);
-                    acc = acc - lds.xb[q][l] * factor;
-                }
-            }
-            lds.xb[q][hl] = acc;
+            double acc[1] = {q == 0 ? B.comx[0] : q == 1 ? B.comx[1] : q == 2 ? B.comx[2]
+                             : q == 3 ? B.comv[0] : q == 4 ? B.comv[1] : B.comv[2]};
+            const bool whds = B.coord == POS_COORD_WHDS;
+            fold_rows<1, true>(B, acc, [&](int, int l) {
+                const double mj = lds.m[l], xq = lds.xb[q][l];
+                const double tx = divr(xq * mj, Rmt);
+                const double tv = xq * (whds ? mj / (m0 + mj) : divr(mj, Rm0));
+                return -(q < 3 ? tx : tv);
+            });
+            lds.xb[q][hl] = acc[0];
         }
         wsync();
         const double nsp[3] = {lds.xb[0][hl], lds.xb[1][hl], lds.xb[2][hl]};
