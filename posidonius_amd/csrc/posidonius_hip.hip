@@ -148,7 +148,11 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
 
 __device__ inline void store_lane(const DevEnsemble &E, const WaveLds &lds, const Lane &B) {
     if (!B.valid()) return;
-    const int b = B.body;
+    // The indices pass through an empty asm so that the field addresses are
+    // formed here again rather than kept from load_lane across the step loop
+    // (where they would be spilled: 94 spilled registers against 35).
+    int b = B.body, s = B.sys;
+    asm volatile("" : "+v"(b), "+v"(s));
     const int nb = E.n_bodies;
     const int lane = B.lane;
 #pragma unroll
@@ -165,7 +169,6 @@ __device__ inline void store_lane(const DevEnsemble &E, const WaveLds &lds, cons
     E.inertia[b] = lds.I[lane];
     E.left_index[b] = B.left_index;
     if (B.k == 0) {
-        const int s = B.sys;
         E.t[s] = B.t;
         E.last_hist[s] = B.last_hist;
         E.iteration[s] = B.iteration;
