@@ -391,11 +391,13 @@ __device__ __forceinline__ void put3(WaveLds &lds, int slot, int lane, double x,
 // terms of FOLD_BATCH bodies of every row are formed first (their LDS loads
 // issue together and one wait covers them), then folded row by row in body
 // order.  Slots past the group's last body, and the host's slot when
-// SKIP_HOST, fold +0: they read the exchange area's zero column WAVE (the
-// load callback indexes the exchange rows).  Every reduction folded this way starts at +0 and adds
-// its terms, so it is never -0 (in round-to-nearest x + y is -0 only when
-// both x and y are); adding +0 to a value that is not -0 leaves it unchanged
-// bit for bit, so each result is the sequential sum over the bodies it names.
+// SKIP_HOST, are given index WAVE: the exchange area's zero column and the
+// zero mass past the last lane, so the callback's term there is +0 (or -0
+// where it negates its terms).  Adding that term leaves the sum unchanged bit
+// for bit, so each result is the sequential sum over the bodies it names:
+// adding -0 changes no value, and adding +0 changes only -0, which a sum
+// that starts at +0 never is (in round-to-nearest x + y is -0 only when both
+// x and y are).  Every caller starts at +0 or adds -0 terms.
 // FOLD_BATCH and the rows per pass of the reaction sums (REACTION_ROWS) are
 // the fastest of the settings measured on the benchmark ensemble: the loads in
 // flight cost registers.
