@@ -84,7 +84,7 @@ __device__ inline void load_lane(const DevEnsemble &E, WaveLds &lds, Lane &B, in
         const bool orb = valid && k != B.host;   // every other body has every orbiting role there
         f = (valid ? F_VALID : 0u) | ((valid && k == B.host) ? F_HOST : 0u) | F_TIDES | F_RF | F_GR | F_SPIN |
             (orb ? (F_TIDES_ROLE | F_RF_ROLE | F_GR_ROLE) : 0u);
-        B.coord = POS_COORD_DEMOCRATIC_HELIOCENTRIC;
+        B.coord = (V & VAR_JACOBI) ? POS_COORD_JACOBI : POS_COORD_DEMOCRATIC_HELIOCENTRIC;
     }
     B.flags = f;
     B.dt = E.dt[s];
@@ -660,9 +660,10 @@ int system_variant(const pos_ensemble_desc *d, int s, int host) {
         for (int b = d->body_offset[s]; b < d->body_offset[s + 1]; ++b) dyn = dyn || evo_dynamical(d->evolution_type[b]);
     if ((effects & POS_EFFECT_DISK) || dyn) v |= VAR_EXT;
     constexpr uint32_t FAST_EFFECTS = POS_EFFECT_TIDES | POS_EFFECT_ROTATIONAL_FLATTENING | POS_EFFECT_GENERAL_RELATIVITY;
-    if (v == GRV_KIDDER && effects == FAST_EFFECTS && implementation == POS_GR_KIDDER1995 &&
-        d->coordinates[s] == POS_COORD_DEMOCRATIC_HELIOCENTRIC && every_body_orbits(d, s, host))
-        v |= VAR_FAST;
+    if (v == GRV_KIDDER && effects == FAST_EFFECTS && implementation == POS_GR_KIDDER1995 && every_body_orbits(d, s, host)) {
+        if (d->coordinates[s] == POS_COORD_DEMOCRATIC_HELIOCENTRIC) v |= VAR_FAST;
+        if (d->coordinates[s] == POS_COORD_JACOBI) v |= VAR_FAST | VAR_JACOBI;
+    }
     return v;
 }
 
@@ -879,6 +880,7 @@ int launch_steps(pos_ens *h, int n_steps, int *launches = nullptr) {
             case 5: launch_steps_variant<5>(h, grid, n_steps, w0, w1); break;
             case 6: launch_steps_variant<6>(h, grid, n_steps, w0, w1); break;
             case 8: launch_steps_variant<8>(h, grid, n_steps, w0, w1); break;
+            case 24: launch_steps_variant<24>(h, grid, n_steps, w0, w1); break;
             default: return fail(POS_E_INVALID, "internal error: unknown kernel variant");
         }
         HIP_TRY(hipGetLastError());
@@ -940,6 +942,7 @@ int launch_effects(pos_ens *h, double *d_a, double *d_l) {
             case 5: hipLaunchKernelGGL(effects_kernel<5>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
             case 6: hipLaunchKernelGGL(effects_kernel<6>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
             case 8: hipLaunchKernelGGL(effects_kernel<8>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
+            case 24: hipLaunchKernelGGL(effects_kernel<24>, g, dim3(64), 0, h->stream, h->E, d_a, d_l, w0, w1); break;
             default: return fail(POS_E_INVALID, "internal error: unknown kernel variant");
         }
         HIP_TRY(hipGetLastError());
@@ -1824,7 +1827,7 @@ int32_t pos_adjoint(pos_ens *h, const pos_adjoint_seed *seed, pos_adjoint_grad *
         if (s + 2 < A.n_seg) HIP_TRY(hipStreamWaitEvent(h->stream, A.swept_ev[buf], 0));
         int r = adjoint_copy_state(h, A.ckpt + (size_t)s * A.state_bytes, false);
         if (r) return r;
-        for (int v = 0; v < N_VARIANTS; ++v) {   // pos_adjoint_record admits variants 0 and VAR_FAST only
+        for (int v = 0; v < N_VARIANTS; ++v) {   // pos_adjoint_record admits variants 0, 8 and 24 only
             const int w0 = h->variant_waves[v][0], w1 = h->variant_waves[v][1];
             if (w1 <= w0) continue;
             const dim3 grid(std::min(h->grid, w1 - w0));
@@ -1836,6 +1839,9 @@ int32_t pos_adjoint(pos_ens *h, const pos_adjoint_seed *seed, pos_adjoint_grad *
             else if (v == (GRV_KIDDER | VAR_FAST))
                 hipLaunchKernelGGL(whfast_record_kernel<GRV_KIDDER | VAR_FAST>, grid, dim3(64), 0, h->stream, Er, seg_len(s), tape,
                                    w0, w1);
+            else if (v == (GRV_KIDDER | VAR_FAST | VAR_JACOBI))
+                hipLaunchKernelGGL(whfast_record_kernel<GRV_KIDDER | VAR_FAST | VAR_JACOBI>, grid, dim3(64), 0, h->stream, Er,
+                                   seg_len(s), tape, w0, w1);
             else
                 return fail(POS_E_INVALID, "internal error: kernel variant without an adjoint");
             HIP_TRY(hipGetLastError());

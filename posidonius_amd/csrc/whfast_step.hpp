@@ -458,12 +458,14 @@ enum GrVariant { GRV_KIDDER = 0, GRV_ANDERSON = 1, GRV_NEWHALL = 2 };
 constexpr int VAR_EXT = 4;
 // VAR_FAST: every system of the launch has exactly tides + flattening + Kidder
 // GR (spins integrated), no wind, disk or evolution, democratic heliocentric
-// coordinates; the flags and the coordinate type are then compile-time
-// constants and the per-system branches fold away (same arithmetic).
+// coordinates (with VAR_JACOBI: Jacobi coordinates); the flags and the
+// coordinate type are then compile-time constants and the per-system branches
+// fold away (same arithmetic).
 constexpr int VAR_FAST = 8;
-constexpr int N_VARIANTS = 16;
+constexpr int VAR_JACOBI = 16;   // only with VAR_FAST
+constexpr int N_VARIANTS = 32;
 constexpr uint32_t EFF_VAR_SHIFT = 16;
-__host__ __device__ __forceinline__ int eff_variant(uint32_t eff) { return (int)((eff >> EFF_VAR_SHIFT) & 15u); }
+__host__ __device__ __forceinline__ int eff_variant(uint32_t eff) { return (int)((eff >> EFF_VAR_SHIFT) & 31u); }
 __host__ __device__ __forceinline__ int eff_gr_variant(uint32_t eff) { return eff_variant(eff) & 3; }
 
 // The effect and role tests of a lane for kernel variant V.  Under VAR_FAST
