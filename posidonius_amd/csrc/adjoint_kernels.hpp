@@ -43,10 +43,13 @@ namespace posdev {
 constexpr int ADJ_ROWS = 28;
 // The adjoint midpoint equation mu = lambda + (h/2) F_y^T mu is iterated
 // until mu changes by less than this (relative, per lane and block) or for
-// at most ADJ_MAX_PASSES passes.  The contraction factor is h |F_y| / 2:
+// at most ADJ_MAX_PASSES passes.  The contraction factor c is h |F_y| / 2:
 // below 1e-7 for the forces, up to ~1e-4 for the tidal spin-down of close
-// planets, so three to five passes are typical.
-constexpr double ADJ_FIXED_POINT_TOL = 1e-14;
+// planets.  After a pass that changed mu by d the remaining error is about
+// c d / (1 - c), so stopping at a change of 1e-10 leaves mu accurate to
+// about 1e-14 relative at c = 1e-4 (two or three passes; a 1e-14 stop took
+// one more pass and gave the same gradients to their last digits).
+constexpr double ADJ_FIXED_POINT_TOL = 1e-10;
 constexpr int ADJ_MAX_PASSES = 12;
 
 struct AdjLds {
