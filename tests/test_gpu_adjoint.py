@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from posidonius_amd import generators as gen
+from tests.spec_tools import move_host
 
 pytestmark = pytest.mark.gpu
 
@@ -228,35 +229,18 @@ def jacobi_workloads():
     """Systems in Jacobi coordinates (whfast.rs:889-963, 1034-1080): the
     star + 3 planet workload switched to Jacobi coordinates, the binary host
     with 1-3 circumbinary planets (3-5 bodies: the host / first body pair
-    left out of the gravity, the indirect term on the planets), and the two
-    mixed with democratic-heliocentric systems so that lanes of both kinds
-    of reverse step share wavefronts."""
+    left out of the gravity, the indirect term on the planets), the same
+    with each primary moved to a later index (the chain then runs host first
+    and the others by index, and the pair left out is the host and the
+    lowest other index), and the two mixed with democratic-heliocentric
+    systems so that lanes of both kinds of reverse step share wavefronts."""
     from posidonius_amd import abi
     from posidonius_amd.case import EnsembleSpec
     four = gen.config4(16)
     four.system["coordinates"][:] = abi.COORD_JACOBI
     mixed = EnsembleSpec.concat([gen.config4(8, seed=31), gen.config3(8, seed=33), gen.config2(8, seed=35)])
     return {"star_3planets": four, "binary_circumbinary": gen.config3(16), "mixed": mixed,
-            "host_moved": host_moved(gen.config3(16, seed=37), np.random.default_rng(37))}
-
-
-def host_moved(spec, rng):
-    """The same systems with each host (body 0) moved to a random index of its
-    system, the other bodies keeping their order: the chain of the Jacobi
-    recurrences then runs host first and the others by index, and the pair
-    left out of the gravity is the host and the lowest other index."""
-    out = spec.copy()
-    perm = np.arange(spec.n_bodies)
-    for s in range(spec.n_systems):
-        b0, b1 = int(spec.body_offset[s]), int(spec.body_offset[s + 1])
-        local = list(range(1, b1 - b0))
-        local.insert(int(rng.integers(1, b1 - b0)), 0)
-        perm[b0:b1] = b0 + np.asarray(local)
-    for k, v in out.body.items():
-        if v is not None:
-            out.body[k] = np.ascontiguousarray(v[perm])
-    out.elements = {k: np.ascontiguousarray(v[perm]) for k, v in spec.elements.items()}
-    return out
+            "host_moved": move_host(gen.config3(16, seed=37), np.random.default_rng(37), first=1)[0]}
 
 
 @pytest.mark.parametrize("name", ["star_3planets", "binary_circumbinary", "mixed", "host_moved"])

@@ -13,6 +13,7 @@ import pytest
 from posidonius_amd import abi, generators as gen
 from posidonius_amd.case import EnsembleSpec
 from tests.oracle_runner import golden_names, load_golden
+from tests.spec_tools import move_host
 
 pytestmark = pytest.mark.gpu
 
@@ -446,24 +447,6 @@ def test_largest_systems_against_oracle(Ensemble, oracle, group_size):
     assert_same_state(out, ref, where=f"16 bodies G={group_size}")
 
 
-def _star_moved(spec, rng):
-    """The same systems with each star moved to a random index of its system
-    (the other bodies keep their order); returns the spec and the map from
-    new to old body index."""
-    out = spec.copy()
-    perm = np.arange(spec.n_bodies)
-    for s in range(spec.n_systems):
-        b0, b1 = int(spec.body_offset[s]), int(spec.body_offset[s + 1])
-        local = list(range(1, b1 - b0))
-        local.insert(int(rng.integers(0, b1 - b0)), 0)
-        perm[b0:b1] = b0 + np.asarray(local)
-    for k, v in out.body.items():
-        if v is not None:
-            out.body[k] = np.ascontiguousarray(v[perm])
-    out.elements = {k: np.ascontiguousarray(v[perm]) for k, v in spec.elements.items()}
-    return out, perm
-
-
 @pytest.mark.parametrize("mapping,group_size", [("packed", -1), ("packed", 0), ("plain", 0), ("packed", -2)])
 def test_fixed_effect_set_with_the_host_anywhere(Ensemble, oracle, mapping, group_size):
     """The fixed-effect-set kernel (tides + flattening + Kidder GR in
@@ -472,7 +455,7 @@ def test_fixed_effect_set_with_the_host_anywhere(Ensemble, oracle, mapping, grou
     lane is not the first lane of its group: bitwise equal to the oracle on
     the same permuted systems."""
     rng = np.random.default_rng(2024)
-    spec, perm = _star_moved(gen.config2(384), rng)
+    spec, perm = move_host(gen.config2(384), rng)
     assert np.any(spec.body["mass"][spec.body_offset[:-1]] < 0.05), "some star must have moved"
     steps = 200
     log(f"host anywhere: {spec.n_systems} systems x {steps} steps, {mapping} G={group_size}")
