@@ -93,6 +93,23 @@ def test_configs_bitwise_against_oracle(Ensemble, oracle, cfg, n, steps):
     assert_same_state(out, ref, where=f"config {cfg}")
 
 
+@pytest.mark.parametrize("group_size", [-2, -1, 0])
+def test_jacobi_fixed_effect_set_all_sizes(Ensemble, oracle, group_size):
+    """The fixed-effect-set kernel in Jacobi coordinates on systems of 2 to
+    16 bodies (two bodies: the one pair is the one the splitting leaves out of
+    the gravity and no body gets the indirect term), mixed in wavefronts or in
+    size classes: bitwise equal to the oracle."""
+    spec = gen.config2(256, seed=47, n_min=2, n_max=abi.MAX_BODIES)
+    spec.system["coordinates"][:] = abi.COORD_JACOBI
+    steps = 200
+    log(f"Jacobi fixed effect set: {spec.n_systems} systems of 2-{abi.MAX_BODIES} bodies x {steps} steps, G={group_size}")
+    ref = oracle.run(spec, steps, threads=THREADS)
+    with Ensemble(spec, mapping="packed", group_size=group_size, steps_per_launch=64) as ens:
+        ens.step(steps)
+        out = ens.get_state()
+    assert_same_state(out, ref, where=f"Jacobi fixed effect set G={group_size}")
+
+
 @pytest.mark.parametrize("group_size,K,graph", [(0, 1, False), (0, 7, False), (8, 64, True), (16, 32, False), (3, 16, True), (-1, 64, True),
                                                (-2, 64, True), (-2, 16, False)])
 def test_launch_mappings_agree(Ensemble, oracle, group_size, K, graph):
