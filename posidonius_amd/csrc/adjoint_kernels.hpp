@@ -55,6 +55,9 @@ constexpr int ADJ_MAX_PASSES = 12;
 struct AdjLds {
     double m[WAVE], I[WAVE], R[WAVE], sg[WAVE], k2[WAVE], kf[WAVE];
     double eta[WAVE];   // Jacobi: the mass of the host and the bodies up to this one in chain order
+    // the last kick's fixed-point correction mu - lambda of each lane (v and
+    // L parts), the next kick's starting guess; each lane uses its own column
+    double wv[3][WAVE], wL[3][WAVE];
     double xb[ADJ_ROWS][WAVE];
 };
 
@@ -221,9 +224,15 @@ __device__ inline void kick_adjoint(AdjLds &L, AdjLane &A, V3 x, V3 v0, V3 L0, V
     const V3 z{0., 0., 0.};
     V3 gx, gv, gL;
     double pg[3];
-    // fixed point of mu = lambda + (h/2) F_y^T mu, starting from mu = lambda;
-    // the passes need only the v and L parts of the product
-    V3 mu_v = lam_v, mu_L = A.spin ? lam_L : z;
+    // fixed point of mu = lambda + (h/2) F_y^T mu, starting from lambda plus
+    // the previous kick's correction (in the reverse sweep the kick before is
+    // the other kick at the same positions or the next one along the orbit,
+    // so the correction changes little and the start is closer than lambda by
+    // about the contraction factor); the passes need only the v and L parts
+    // of the product
+    const V3 wv{L.wv[0][lane], L.wv[1][lane], L.wv[2][lane]};
+    const V3 wL{L.wL[0][lane], L.wL[1][lane], L.wL[2][lane]};
+    V3 mu_v = lam_v + wv, mu_L = A.spin ? lam_L + wL : z;
     for (int pass = 0;; ++pass) {
         effects_vjp_pass<false>(L, A, C, P, in, mu_v, mu_L, gx, gv, gL, pg);
         const V3 nv = lam_v + (0.5 * h) * gv;
@@ -232,6 +241,11 @@ __device__ inline void kick_adjoint(AdjLds &L, AdjLane &A, V3 x, V3 v0, V3 L0, V
         mu_v = nv;
         mu_L = nL;
         if (pass + 1 >= ADJ_MAX_PASSES || group_all(L, A, conv)) break;
+    }
+    {
+        const V3 cv = mu_v - lam_v, cL = A.spin ? mu_L - lam_L : z;
+        L.wv[0][lane] = cv.x; L.wv[1][lane] = cv.y; L.wv[2][lane] = cv.z;
+        L.wL[0][lane] = cL.x; L.wL[1][lane] = cL.y; L.wL[2][lane] = cL.z;
     }
     // the x and parameter parts (and, without spin, the L part) at the fixed point
     effects_vjp_pass<true>(L, A, C, P, in, mu_v, mu_L, gx, gv, gL, pg);
@@ -655,6 +669,7 @@ __device__ inline void adjoint_load(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     A.eta_le = eta;
     A.eta_lt = eta - A.m;
     L.eta[lane] = eta;
+    for (int q = 0; q < 3; ++q) { L.wv[q][lane] = 0.; L.wL[q][lane] = 0.; }
     __syncthreads();
 }
 
