@@ -234,6 +234,23 @@ def test_benchmark_ensemble_at_horizon(Ensemble, oracle):
     log(f"{int(np.count_nonzero(out['current_iteration'] < steps))} systems stopped early (fatal status)")
 
 
+def test_binary_hosts_at_their_horizon(Ensemble, oracle):
+    """The binary-host workload (2 stars + 1-3 circumbinary planets, Jacobi
+    coordinates, the Jacobi fixed-effect-set kernel) over its full 10^5
+    steps, on 64 of its systems: every system equals the oracle bit for bit,
+    status and Kepler-solver flags included."""
+    spec = gen.config3(64)
+    steps = 100_000
+    log(f"binary hosts: {spec.n_systems} systems x {steps} steps")
+    with Ensemble(spec, mapping="packed", steps_per_launch=64, use_graph=True) as ens:
+        ens.step(steps)
+        out = ens.get_state()
+    log(f"oracle on {THREADS} threads")
+    ref = oracle.run(spec, steps, threads=THREADS)
+    assert_same_state(out, ref, where="binary hosts at 1e5 steps")
+    log(f"status bits set: {sorted(set(int(x) for x in out['status']))}")
+
+
 def gr_variant_ensemble():
     """The uneven-size ensemble and the Jacobi binary-host ensemble, each under
     every general-relativity implementation (Kidder, Anderson, Newhall,
