@@ -234,6 +234,21 @@ def test_benchmark_ensemble_at_horizon(Ensemble, oracle):
     log(f"{int(np.count_nonzero(out['current_iteration'] < steps))} systems stopped early (fatal status)")
 
 
+@pytest.mark.parametrize("cfg,n", [(1, 256), (2, 512)])
+def test_configs_at_their_horizon(Ensemble, oracle, cfg, n):
+    """The uniform (3 bodies) and the uneven (2-12 bodies) ensembles over
+    their full 10^4 steps, on a subset of their systems, under the
+    benchmark's launch shape: every system equals the oracle bit for bit."""
+    spec = gen.CONFIGS[cfg](n)
+    steps = 10_000
+    log(f"config {cfg}: {spec.n_systems} systems x {steps} steps")
+    with Ensemble(spec, mapping="packed", group_size=-2, steps_per_launch=64, use_graph=True) as ens:
+        ens.step(steps)
+        out = ens.get_state()
+    ref = oracle.run(spec, steps, threads=THREADS)
+    assert_same_state(out, ref, where=f"config {cfg} at 1e4 steps")
+
+
 def test_binary_hosts_at_their_horizon(Ensemble, oracle):
     """The binary-host workload (2 stars + 1-3 circumbinary planets, Jacobi
     coordinates, the Jacobi fixed-effect-set kernel) over its full 10^5
