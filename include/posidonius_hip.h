@@ -359,9 +359,10 @@ int32_t pos_history_drain(pos_ens *ens, pos_history_record *out, int64_t max_rec
 
 /*
  * Discrete adjoint (the differentiable form).  Supported systems: democratic
- * heliocentric or Jacobi coordinates, any of tides / rotational flattening /
- * Kidder 1995 GR, with or without wind and evolution tables (radius and
- * radius of gyration following time), no disk and no evolving dissipation.
+ * heliocentric, Jacobi or WHDS coordinates, any of tides / rotational
+ * flattening / Kidder 1995 GR, with or without wind and evolution tables
+ * (radius and radius of gyration following time), no disk and no evolving
+ * dissipation.
  * No counterpart in the reference,
  * which has no derivatives; it differentiates the step of whfast.rs:235-305.
  *

@@ -75,6 +75,8 @@ struct AdjLane {
     int lane, gbase, k, n, host, sys, body;
     bool valid, is_host, tides, flat, gr, spin, wind, jacobi;
     bool evolving;        // evolution tables: radius and moment of inertia follow time
+    bool whds;            // WHDS coordinates
+    double fv, wj;        // WHDS: velocity factor (m_h + m) / m_h and jump weight m / (m_h + m)
     int chain;            // position in the chain of the Jacobi recurrences: host 0, then the others in index order
     double eta_le, eta_lt;   // mass of the chain up to this body, with and without it
     double dt, m, I, R, sg, k2, kf, mh, Ih, M;
@@ -278,9 +280,14 @@ __device__ __forceinline__ void step_radii(const TapeRef &T, AdjLds &L, AdjLane 
     if (A.wind) A.wshape = sqrt(A.R / R_SUN * 1. / A.m);
 }
 
-// Reverse of step `row` of the segment: A.l* hold the cotangents of the
+// Reverse of step `row` of the segment for democratic heliocentric
+// coordinates or, with WHDS, for WHDS coordinates (whfast.rs:973-1023,
+// 1090-1155, 495-556, 558-625): the same step with the velocities scaled by
+// f = (m_h + m) / m_h, Kepler drifts about m_h + m, a jump weighted by
+// w = m / (m_h + m) without the body's own term, and the interaction kick
+// scaled by f.  A.l* hold the cotangents of the
 // state after the step on entry and before it on exit.
-template <bool EVO>
+template <bool EVO, bool WHDS>
 __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, int row) {
     const TapeRef &T = Ar.T;
     const int lane = A.lane;
@@ -296,7 +303,7 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     }
     step_radii<EVO>(T, L, A, row);
     const double h = 0.5 * A.dt, dt = A.dt;
-    const double mu = posadj::G_K2 * A.mh;
+    const double mu = posadj::G_K2 * (WHDS ? A.mh + A.m : A.mh);
     const bool orb = A.orbiting();
     const int hl = A.hl();
 
@@ -307,14 +314,16 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     const V3 Xc = (1. / A.M) * gsum3m(L, A, 0, false);
     const V3 V = (1. / A.M) * gsum3m(L, A, 3, false);
     const V3 Q0 = orb ? x0 - get3(L, 0, hl) : z;
-    const V3 P0 = orb ? v1 - V : z;
+    const V3 P0 = orb ? (WHDS ? A.fv * (v1 - V) : v1 - V) : z;
     wsync();
     V3 Q1 = z, P1 = z;
     if (orb) posadj::kepler_forward_x(mu, h, X1, Q0, P0, Q1, P1);
     const V3 Xc1 = Xc + h * V;
-    if (orb) put3(L, 0, lane, P1);
+    if (orb) put3(L, 0, lane, WHDS ? A.wj * P1 : P1);
     wsync();
-    const V3 Q1p = orb ? Q1 + (h / A.mh) * gsum3m(L, A, 0, true) : z;
+    const V3 Q1p = !orb ? z
+                 : WHDS ? Q1 + h * (gsum3(L, A, 0, true) - A.wj * P1)
+                        : Q1 + (h / A.mh) * gsum3m(L, A, 0, true);
     wsync();
     if (orb) put3(L, 0, lane, Q1p);
     wsync();
@@ -334,17 +343,20 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
             acc += (-posadj::G_K2 * L.m[l] * id * id * id) * d;
         }
     }
-    const V3 P1pp = P1 + dt * acc;
+    const V3 P1pp = P1 + (WHDS ? dt * A.fv : dt) * acc;
     wsync();
-    if (orb) put3(L, 0, lane, P1pp);
+    if (orb) put3(L, 0, lane, WHDS ? A.wj * P1pp : P1pp);
     wsync();
-    const V3 Q2 = orb ? Q1p + (h / A.mh) * gsum3m(L, A, 0, true) : z;
+    const V3 Q2 = !orb ? z
+                : WHDS ? Q1p + h * (gsum3(L, A, 0, true) - A.wj * P1pp)
+                       : Q1p + (h / A.mh) * gsum3m(L, A, 0, true);
     V3 Q3 = z, P3 = z;
     if (orb) posadj::kepler_forward_x(mu, h, X2, Q2, P1pp, Q3, P3);
     wsync();
-    if (orb) put3(L, 0, lane, P3);
+    if (orb) put3(L, 0, lane, WHDS ? A.wj * P3 : P3);
     wsync();
-    const V3 v2p = orb ? P3 + V : V - (1. / A.mh) * gsum3m(L, A, 0, true);
+    const V3 v2p = WHDS ? (orb ? (1. / A.fv) * P3 + V : V - gsum3(L, A, 0, true))
+                        : (orb ? P3 + V : V - (1. / A.mh) * gsum3m(L, A, 0, true));
     wsync();
 
     // ---------------- reverse ----------------
@@ -357,7 +369,7 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     const V3 lvh = get3(L, 3, hl);
     const V3 Vb0 = lvh + gsum3(L, A, 3, true);
     const V3 Q3b = orb ? ld3(A.lx) - (A.m / A.M) * xh_tot : z;
-    const V3 P3b = orb ? ld3(A.lv) - (A.m / A.mh) * lvh : z;
+    const V3 P3b = !orb ? z : WHDS ? (1. / A.fv) * ld3(A.lv) - A.wj * lvh : ld3(A.lv) - (A.m / A.mh) * lvh;
     wsync();
     // second Kepler drift and the centre-of-mass drift
     V3 Q2b = z, P1ppb = z;
@@ -367,12 +379,12 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     // second jump
     if (orb) put3(L, 0, lane, Q2b);
     wsync();
-    if (orb) P1ppb += (h * A.m / A.mh) * gsum3(L, A, 0, true);
+    if (orb) P1ppb += WHDS ? (h * A.wj) * (gsum3(L, A, 0, true) - Q2b) : (h * A.m / A.mh) * gsum3(L, A, 0, true);
     V3 Q1pb = Q2b;
     wsync();
     // interaction and planet-planet gravity
     V3 P1b = P1ppb;
-    const V3 ab = dt * P1ppb;
+    const V3 ab = (WHDS ? dt * A.fv : dt) * P1ppb;
     if (A.valid) { put3(L, 0, lane, orb ? ab : z); put3(L, 3, lane, xg); }
     wsync();
     V3 xgb = z;
@@ -395,7 +407,7 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     // first jump
     if (orb) put3(L, 0, lane, Q1pb);
     wsync();
-    if (orb) P1b += (h * A.m / A.mh) * gsum3(L, A, 0, true);
+    if (orb) P1b += WHDS ? (h * A.wj) * (gsum3(L, A, 0, true) - Q1pb) : (h * A.m / A.mh) * gsum3(L, A, 0, true);
     wsync();
     // first Kepler drift and the centre-of-mass drift
     V3 Q0b = z, P0b = z;
@@ -403,13 +415,14 @@ __device__ inline void adjoint_step(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     const V3 Xcb = Xc1b;
     Vb += h * Xc1b;
     // T of the first drift
-    if (orb) { put3(L, 0, lane, Q0b); put3(L, 3, lane, P0b); }
+    const V3 v1b = WHDS ? A.fv * P0b : P0b;   // P0 = f (v1 - V), f = 1 in heliocentric coordinates
+    if (orb) { put3(L, 0, lane, Q0b); put3(L, 3, lane, v1b); }
     wsync();
     const V3 SQ = gsum3(L, A, 0, true), SP = gsum3(L, A, 3, true);
     const V3 Vtot = Vb - SP;
     if (A.valid) {
         st3(A.lx, (orb ? Q0b : -SQ) + (A.m / A.M) * Xcb);
-        st3(A.lv, (orb ? P0b : z) + (A.m / A.M) * Vtot);
+        st3(A.lv, (orb ? v1b : z) + (A.m / A.M) * Vtot);
     }
     wsync();
     // first kick (at x0, between (v0, L0) and (v1, L1))
@@ -662,6 +675,9 @@ __device__ inline void adjoint_load(const AdjArgs &Ar, AdjLds &L, AdjLane &A, in
     for (int j = 0; j < A.n; ++j) M += L.m[A.gbase + j];
     A.M = M;
     A.jacobi = E.coord[s] == POS_COORD_JACOBI;
+    A.whds = E.coord[s] == POS_COORD_WHDS;
+    A.fv = (A.mh + A.m) / A.mh;
+    A.wj = A.m / (A.mh + A.m);
     A.chain = chain_of(A, A.k);
     double eta = 0.;
     for (int j = 0; j < A.n; ++j)

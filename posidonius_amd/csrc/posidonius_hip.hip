@@ -227,12 +227,12 @@ void whfast_record_kernel(DevEnsemble E, int n_steps, TapeRef T, int wave_begin,
     }
 }
 
-// Reverse sweep over one recorded segment, last step first.  JACOBI: the
-// ensemble holds systems in Jacobi coordinates; EVO: systems with evolution
-// tables.  Without them the Jacobi reverse step and the per-step tape reads
-// of the radii are compiled out, which keeps the heliocentric sweep's
-// register allocation as it is alone.
-template <bool JACOBI, bool EVO>
+// Reverse sweep over one recorded segment, last step first.  ALT: the
+// ensemble holds systems in Jacobi or WHDS coordinates; EVO: systems with
+// evolution tables.  Without them the reverse steps of the other coordinate
+// types and the per-step tape reads of the radii are compiled out, which
+// keeps the heliocentric sweep's register allocation as it is alone.
+template <bool ALT, bool EVO>
 __global__ __launch_bounds__(64) void adjoint_segment_kernel(AdjArgs Ar) {
     __shared__ AdjLds L;
     for (int wave = blockIdx.x; wave < Ar.E.n_waves; wave += gridDim.x) {
@@ -242,8 +242,9 @@ __global__ __launch_bounds__(64) void adjoint_segment_kernel(AdjArgs Ar) {
             // uniform within the group, not the wavefront: correct only because
             // the workgroup is this one wavefront (see whfast_steps_kernel)
             if (Ar.seg_start + j >= A.done) continue;
-            if (JACOBI && A.jacobi) adjoint_step_jacobi<EVO>(Ar, L, A, j);
-            else adjoint_step<EVO>(Ar, L, A, j);
+            if (ALT && A.jacobi) adjoint_step_jacobi<EVO>(Ar, L, A, j);
+            else if (ALT && A.whds) adjoint_step<EVO, true>(Ar, L, A, j);
+            else adjoint_step<EVO, false>(Ar, L, A, j);
         }
         adjoint_store(Ar, A);
         __syncthreads();
@@ -1708,8 +1709,7 @@ int32_t pos_adjoint_record(pos_ens *h, int64_t n_steps, int32_t segment_steps) {
     if (segment_steps < 0) return fail(POS_E_INVALID, "segment_steps must be >= 0");
     for (int ds = 0; ds < h->n_systems; ++ds) {
         const std::string where = "system " + std::to_string(h->sys_perm[ds]) + ": ";
-        if (h->coord_dev[ds] != POS_COORD_DEMOCRATIC_HELIOCENTRIC && h->coord_dev[ds] != POS_COORD_JACOBI)
-            return fail(POS_E_UNSUPPORTED, where + "the adjoint supports democratic-heliocentric and Jacobi coordinates only");
+
         if (h->eff_dev[ds] & POS_EFFECT_DISK)
             return fail(POS_E_UNSUPPORTED, where + "the adjoint does not support the disk");
         if ((h->variant_dev[ds] & 3) != GRV_KIDDER)
@@ -1849,10 +1849,11 @@ int32_t pos_adjoint(pos_ens *h, const pos_adjoint_seed *seed, pos_adjoint_grad *
         HIP_TRY(hipEventRecord(A.recorded_ev[buf], h->stream));
         return POS_OK;
     };
-    const bool jacobi = std::find(h->coord_dev.begin(), h->coord_dev.end(), POS_COORD_JACOBI) != h->coord_dev.end();
+    const bool alt = std::any_of(h->coord_dev.begin(), h->coord_dev.end(),
+                                 [](int c) { return c != POS_COORD_DEMOCRATIC_HELIOCENTRIC; });
     const bool evo = std::any_of(h->eff_dev.begin(), h->eff_dev.end(),
                                  [](uint32_t e) { return (e & POS_EFFECT_EVOLUTION) != 0; });
-    void (*sweep_kernel)(AdjArgs) = jacobi ? (evo ? adjoint_segment_kernel<true, true> : adjoint_segment_kernel<true, false>)
+    void (*sweep_kernel)(AdjArgs) = alt ? (evo ? adjoint_segment_kernel<true, true> : adjoint_segment_kernel<true, false>)
                                     : (evo ? adjoint_segment_kernel<false, true> : adjoint_segment_kernel<false, false>);
     // the sweep stream must also see the seeds uploaded on the handle's stream
     HIP_TRY(hipEventRecord(A.swept_ev[0], h->stream));

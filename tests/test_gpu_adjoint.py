@@ -225,8 +225,11 @@ def test_adjoint_with_evolution_equals_tangent_linear(Ensemble, oracle):
         check_against_tangent(ens, spec, rng, d, tan, "evolution")
 
 
-def jacobi_workloads():
-    """Systems in Jacobi coordinates (whfast.rs:889-963, 1034-1080): the
+def coordinate_workloads():
+    """Systems in WHDS coordinates (whfast.rs:973-1023, 1090-1155, 495-556:
+    the star + 3 planet workload, systems of 2-12 bodies, and the three
+    coordinate types mixed in wavefronts) and in Jacobi coordinates
+    (whfast.rs:889-963, 1034-1080): the
     star + 3 planet workload switched to Jacobi coordinates, the binary host
     with 1-3 circumbinary planets (3-5 bodies: the host / first body pair
     left out of the gravity, the indirect term on the planets), the same
@@ -239,21 +242,31 @@ def jacobi_workloads():
     four = gen.config4(16)
     four.system["coordinates"][:] = abi.COORD_JACOBI
     mixed = EnsembleSpec.concat([gen.config4(8, seed=31), gen.config3(8, seed=33), gen.config2(8, seed=35)])
+    whds4 = gen.config4(16, seed=39)
+    whds4.system["coordinates"][:] = abi.COORD_WHDS
+    whds_sizes = gen.config2(24, seed=41, n_min=2, n_max=12)
+    whds_sizes.system["coordinates"][:] = abi.COORD_WHDS
+    all3 = EnsembleSpec.concat([gen.config4(8, seed=43), gen.config3(8, seed=45), whds4])
     return {"star_3planets": four, "binary_circumbinary": gen.config3(16), "mixed": mixed,
-            "host_moved": move_host(gen.config3(16, seed=37), np.random.default_rng(37), first=1)[0]}
+            "host_moved": move_host(gen.config3(16, seed=37), np.random.default_rng(37), first=1)[0],
+            "whds_star_3planets": whds4, "whds_2_12_bodies": whds_sizes, "all_three_coordinates": all3}
 
 
-@pytest.mark.parametrize("name", ["star_3planets", "binary_circumbinary", "mixed", "host_moved"])
+@pytest.mark.parametrize("name", ["star_3planets", "binary_circumbinary", "mixed", "host_moved", "whds_star_3planets",
+                                  "whds_2_12_bodies", "all_three_coordinates"])
 @pytest.mark.parametrize("steps,segment", [(300, 100), (3_000, 1000)])
-def test_adjoint_in_jacobi_coordinates_equals_tangent_linear(Ensemble, oracle, name, steps, segment):
-    """Jacobi coordinates: the reverse step's transforms, Kepler drifts of the
-    chain's masses, skipped pair and indirect term against the tangent-linear
-    run, every seed block and direction to 1e-6 relative."""
+def test_adjoint_in_other_coordinates_equals_tangent_linear(Ensemble, oracle, name, steps, segment):
+    """Jacobi coordinates (the reverse step's transforms, Kepler drifts of the
+    chain's masses, skipped pair and indirect term) and WHDS coordinates (the
+    velocity factors, Kepler drifts about the pair's mass, the weighted jump
+    and the scaled interaction kick) against the tangent-linear run, every
+    seed block and direction to 1e-6 relative."""
     from posidonius_amd import abi
-    spec = jacobi_workloads()[name]
+    spec = coordinate_workloads()[name]
     n_jacobi = int(np.sum(spec.system["coordinates"] == abi.COORD_JACOBI))
-    print(f"adjoint in Jacobi coordinates: {name}, {spec.n_systems} systems ({n_jacobi} Jacobi) x {steps} steps, "
-          f"segments of {segment}", flush=True)
+    n_whds = int(np.sum(spec.system["coordinates"] == abi.COORD_WHDS))
+    print(f"adjoint in other coordinates: {name}, {spec.n_systems} systems ({n_jacobi} Jacobi, {n_whds} WHDS) x {steps} "
+          f"steps, segments of {segment}", flush=True)
     rng = np.random.default_rng(41)
     d = directions(spec, rng)
     ref, tan = oracle.tangent(spec, steps, d, threads=max(1, min(16, os.cpu_count() or 1)))
@@ -262,7 +275,7 @@ def test_adjoint_in_jacobi_coordinates_equals_tangent_linear(Ensemble, oracle, n
         fin = ens.get_state()
         for k in BLOCKS:
             np.testing.assert_array_equal(fin[k], ref[k], err_msg=f"recorded forward run differs in {k}")
-        check_against_tangent(ens, spec, rng, d, tan, f"jacobi {name}")
+        check_against_tangent(ens, spec, rng, d, tan, f"coordinates {name}")
 
 
 def test_adjoint_is_the_same_for_every_launch_mapping(Ensemble):

@@ -312,19 +312,11 @@ def test_adjoint_rejects_other_gr_variants(Ensemble):
             ens.adjoint_record(10, 5)
 
 
-@pytest.mark.parametrize("what", ["whds", "disk"])
-def test_adjoint_rejects_what_it_does_not_differentiate(Ensemble, what):
-    """The adjoint refuses WHDS coordinates and the disk with a clear error
-    (heliocentric and Jacobi systems are accepted)."""
-    if what == "whds":
-        spec = gen.config2(4, seed=3)
-        spec.system["coordinates"][:] = abi.COORD_WHDS
-        match = "democratic-heliocentric and Jacobi coordinates only"
-    else:
-        spec, _ = load_golden("test_disk-enabled_disk")
-        match = "does not support the disk"
+def test_adjoint_rejects_the_disk(Ensemble):
+    """The adjoint refuses the disk with a clear error."""
+    spec, _ = load_golden("test_disk-enabled_disk")
     with Ensemble(spec) as ens:
-        with pytest.raises(Exception, match=match):
+        with pytest.raises(Exception, match="does not support the disk"):
             ens.adjoint_record(10, 5)
 
 
