@@ -19,6 +19,11 @@ apsidal motion comes from the GR accelerations alone:
   equations (the device reproduces them bit for bit), so the oracle's
   advance must equal that of an independent integration of the same
   equations, and exceed Einstein's by the amount those equations give.
+
+Apsidal motion under the star's rotational flattening
+(rotational_flattening/common.rs:203-237) has the closed form of a J2
+quadrupole for an orbit in the star's equatorial plane; the oracle must
+reproduce it and an independent integration of the reference's force.
 """
 import numpy as np
 import pytest
@@ -121,3 +126,69 @@ def test_kidder_apsidal_advance(oracle, a, e, M):
     # and their second-order term adds an advance of order G M / a (AU/day units)
     excess = as_written / first - 1
     assert 1.0 < excess / (mu / a) < 4.0, excess
+
+
+def flattening_relative_acceleration(r, M, m, k2f, omega, R):
+    """Relative acceleration under the star's rotational flattening as the
+    reference writes it for an orbit in the star's equatorial plane
+    (oblate_spheroid.rs:8-60, rotational_flattening/common.rs:203-237: the
+    force -3 C x / d^5 with C = m k2f Omega^2 R^5 / 6 on the planet, -m/M of
+    it on the host), plus Newton."""
+    mu = gen.G * (M + m)
+    d = np.linalg.norm(r)
+    C = m * k2f * omega ** 2 * R ** 5 / 6.0
+    force = -3.0 * C / d ** 5 * r
+    return -mu / d ** 3 * r + force / m * (1 + m / M)
+
+
+@pytest.mark.parametrize("a,e", [(0.03, 0.2), (0.05, 0.1)])
+def test_rotational_flattening_apsidal_advance(oracle, a, e):
+    """The star's rotational flattening acts on an orbit in its equatorial
+    plane as a quadrupole J2 = k2f Omega^2 R^3 / (3 G M) (the reference's
+    force is -(k2f Omega^2 R^5 / 2) x / d^5 per unit mass, the gradient of the
+    J2 potential), which advances the periapsis at (3/2) n J2 (R / p)^2.  The
+    planet's own fluid Love number is zero, the spins are aligned with the
+    orbit normal (no torque, so they stay constant), only flattening is on."""
+    M, m = 1.0, gen.M_EARTH
+    R, k2f, omega = gen.R_SUN, 0.03, 2 * np.pi / 2.0
+    mu = gen.G * (M + m)
+    n = np.sqrt(mu / a ** 3)
+    period = 2 * np.pi / n
+    orbits, steps_per_orbit = 1000, 100
+    p, v = gen.cartesian_from_elements(mu, np.array([a * (1 - e)]), np.array([e]), np.array([0.0]),
+                                       np.array([1.1]), np.array([0.0]), np.array([2.0]))
+    pos = np.zeros((1, 2, 3)); vel = np.zeros((1, 2, 3))
+    pos[0, 1], vel[0, 1] = p[0], v[0]
+    roles = np.array([[abi.ROLE_FLAT_CENTRAL, abi.ROLE_FLAT_ORBITING]], dtype=np.uint32)
+    spin = np.zeros((1, 2, 3))
+    spin[0, 0, 2], spin[0, 1, 2] = omega, 2 * np.pi / 1.0
+    zeros = np.zeros((1, 2))
+    spec = gen.build_spec([2], np.array([[M, m]]), np.array([[R, gen.R_EARTH]]), np.array([[0.07, 0.33]]),
+                          pos, vel, spin, roles, zeros, np.ones((1, 2)), zeros, np.array([[k2f, 0.0]]),
+                          period / steps_per_orbit, abi.COORD_DEMOCRATIC_HELIOCENTRIC,
+                          effects=abi.EFFECT_ROTATIONAL_FLATTENING)
+    out = oracle.run(spec, orbits * steps_per_orbit, threads=1)
+    assert out["status"][0] == 0
+    t_end = out["current_time"][0]
+    np.testing.assert_allclose(out["spin"][0], spin[0, 0], rtol=1e-12)
+
+    x0, v0 = spec.body["position"], spec.body["velocity"]
+    r0, u0 = x0[1] - x0[0], v0[1] - v0[0]
+    x1, v1 = out["position"], out["velocity"]
+    oracle_advance = advance(r0, u0, x1[1] - x1[0], v1[1] - v1[0], mu)
+
+    def rhs(t, y):
+        return np.concatenate([y[3:], flattening_relative_acceleration(y[:3], M, m, k2f, omega, R)])
+    sol = solve_ivp(rhs, (0.0, t_end), np.concatenate([r0, u0]), rtol=1e-12, atol=1e-15, method="DOP853")
+    integrated = advance(r0, u0, sol.y[:3, -1], sol.y[3:, -1], mu)
+
+    J2 = k2f * omega ** 2 * R ** 3 / (3 * gen.G * M)
+    closed = 1.5 * n * J2 * (R / (a * (1 - e ** 2))) ** 2 * t_end
+    print(f"a={a} e={e}: J2 {J2:.3e}, closed form {closed:.6e}, integrated {integrated:.6e}, "
+          f"oracle {oracle_advance:.6e} rad over {t_end / period:.1f} orbits")
+    # the reference's force integrated independently gives the J2 advance
+    # (to first order in J2 (R/a)^2 ~ 1e-6 and the osculating elements'
+    # periodic terms, ~J2 (R/a)^2 / e, a few 1e-4 of it here)
+    assert abs(integrated / closed - 1) < 2e-3, (integrated, closed)
+    # the oracle integrates the same force
+    assert abs(oracle_advance / integrated - 1) < 1e-4, (oracle_advance, integrated)
