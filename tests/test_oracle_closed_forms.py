@@ -24,6 +24,11 @@ Apsidal motion under the star's rotational flattening
 (rotational_flattening/common.rs:203-237) has the closed form of a J2
 quadrupole for an orbit in the star's equatorial plane; the oracle must
 reproduce it and an independent integration of the reference's force.
+
+Constant-time-lag tides (Hut 1981; constant_time_lag.rs:206-332) relax a
+planet's spin to the mean motion of its circular orbit exponentially, with
+a time constant in closed form; the oracle's orbit and spin must follow an
+independent integration of Hut's force and torque.
 """
 import numpy as np
 import pytest
@@ -192,3 +197,86 @@ def test_rotational_flattening_apsidal_advance(oracle, a, e):
     assert abs(integrated / closed - 1) < 2e-3, (integrated, closed)
     # the oracle integrates the same force
     assert abs(oracle_advance / integrated - 1) < 1e-4, (oracle_advance, integrated)
+
+
+def hut_tidal_force(x, u, spin, M, k2dt, R):
+    """Dissipative part of the constant-time-lag tidal force that a star of
+    mass M raises on a planet of radius R and spin vector spin (Hut 1981,
+    eq. 9; the reference's constant_time_lag.rs:206-332 with its dissipation
+    factor sigma = 2 G k2 dt / (3 R^5), as its example cases set it)."""
+    d2 = x @ x
+    return -3 * gen.G * M ** 2 * k2dt * R ** 5 / d2 ** 5 * (2 * x * (x @ u) + d2 * (np.cross(x, spin) + u))
+
+
+@pytest.mark.parametrize("spin_over_n", [3.0, 0.2])
+def test_constant_time_lag_spin_relaxation(oracle, spin_over_n):
+    """A planet on a circular orbit with its spin along the orbit normal and
+    only its own tide dissipating: the tidal torque -3 G M^2 k2 dt R^5 (Omega -
+    n) / a^6 relaxes the spin to the mean motion n exponentially with time
+    constant tau = rg2 m a^6 / (3 G M^2 k2 dt R^3) (the orbit's angular
+    momentum is 1e5 times the spin's, so a and n stay fixed to 1e-5).  An
+    independent integration of the orbit and the spin under Hut's force
+    follows the oracle."""
+    M, m, R, rg2, a = 0.08, gen.M_EARTH, gen.R_EARTH, 0.33, 0.01
+    k2dt = 10 * 2.465278e-3
+    mu = gen.G * (M + m)
+    n = np.sqrt(mu / a ** 3)
+    period = 2 * np.pi / n
+    tau = rg2 * m * a ** 6 / (3 * gen.G * M ** 2 * k2dt * R ** 3)
+    steps_per_orbit = 50
+    steps = int(round(1.5 * tau / (period / steps_per_orbit)))
+    p, v = gen.cartesian_from_elements(mu, np.array([a]), np.array([0.0]), np.array([0.0]),
+                                       np.array([0.6]), np.array([0.0]), np.array([1.3]))
+    pos = np.zeros((1, 2, 3)); vel = np.zeros((1, 2, 3))
+    pos[0, 1], vel[0, 1] = p[0], v[0]
+    normal = np.cross(p[0], v[0]); normal /= np.linalg.norm(normal)
+    spin = np.zeros((1, 2, 3))
+    spin[0, 0] = 2 * np.pi / 30.0 * normal
+    spin[0, 1] = spin_over_n * n * normal
+    roles = np.array([[abi.ROLE_TIDES_CENTRAL, abi.ROLE_TIDES_ORBITING]], dtype=np.uint32)
+    sigma = np.array([[0.0, 2 * gen.G * k2dt / (3 * R ** 5)]])
+    zeros = np.zeros((1, 2))
+    spec = gen.build_spec([2], np.array([[M, m]]), np.array([[0.1 * gen.R_SUN, R]]), np.array([[0.2, rg2]]),
+                          pos, vel, spin, roles, sigma, np.ones((1, 2)), zeros, zeros,
+                          period / steps_per_orbit, abi.COORD_DEMOCRATIC_HELIOCENTRIC, effects=abi.EFFECT_TIDES)
+    out = oracle.run(spec, steps, threads=1)
+    assert out["status"][0] == 0
+    t_end = out["current_time"][0]
+    np.testing.assert_array_equal(out["spin"][0], spin[0, 0])
+    # the state is the angular momentum: the spin vector a step leaves behind
+    # is the one of the last midpoint iteration (L a quarter step before the
+    # step's end), as in the reference, which recomputes the spin from L
+    # inside the additional effects (whfast.rs:322-466) and before it writes a
+    # snapshot (whfast.rs:239-243), not after the step
+    inertia = rg2 * m * R ** 2
+    oracle_spin = out["angular_momentum"][1] / inertia
+    lag = np.linalg.norm(out["spin"][1]) / np.linalg.norm(oracle_spin) - 1
+    assert 0 < lag * np.sign(spin_over_n - 1) < 1e-3
+
+    # closed form: the spin stays on the normal and relaxes to n
+    omega = oracle_spin @ normal
+    x1, v1 = out["position"], out["velocity"]
+    r1, u1 = x1[1] - x1[0], v1[1] - v1[0]
+    n_end = np.sqrt(mu / (1 / (2 / np.linalg.norm(r1) - u1 @ u1 / mu)) ** 3)
+    rate = -np.log((omega - n_end) / ((spin_over_n - 1) * n)) / t_end
+    print(f"spin/n={spin_over_n}: {t_end / tau:.2f} tau, relaxation rate x tau {rate * tau:.6f}, "
+          f"n changed by {n_end / n - 1:.2e}")
+    assert np.linalg.norm(np.cross(oracle_spin, normal)) < 1e-9 * np.linalg.norm(oracle_spin)
+    assert abs(rate * tau - 1) < 1e-3
+
+    # the oracle's orbit and spin against an independent integration of Hut's force
+    x0, v0 = spec.body["position"], spec.body["velocity"]
+
+    def rhs(t, y):
+        x, u, L = y[:3], y[3:6], y[6:]
+        F = hut_tidal_force(x, u, L / inertia, M, k2dt, R)
+        acc = -mu / np.linalg.norm(x) ** 3 * x + F * (1 / m + 1 / M)
+        return np.concatenate([u, acc, -np.cross(x, F)])
+    y0 = np.concatenate([x0[1] - x0[0], v0[1] - v0[0], inertia * spin[0, 1]])
+    sol = solve_ivp(rhs, (0.0, t_end), y0, rtol=1e-12, atol=1e-16, method="DOP853")
+    integrated_spin = sol.y[6:, -1] / inertia
+    # (the difference is the scheme's second-order error: 1.3e-7 of Omega - n at
+    # 50 steps per orbit, a quarter of that at 100)
+    np.testing.assert_allclose(oracle_spin, integrated_spin, rtol=0, atol=1e-7 * np.linalg.norm(integrated_spin))
+    # the orbit: same radius (circular) and same phase to the step's error
+    np.testing.assert_allclose(r1, sol.y[:3, -1], rtol=0, atol=1e-7 * a)
