@@ -29,6 +29,10 @@ Constant-time-lag tides (Hut 1981; constant_time_lag.rs:206-332) relax a
 planet's spin to the mean motion of its circular orbit exponentially, with
 a time constant in closed form; the oracle's orbit and spin must follow an
 independent integration of Hut's force and torque.
+
+The two other GR implementations (Anderson et al. 1975, Newhall et al. 1983)
+give Einstein's advance, and the wind's magnetic braking (wind.rs:71-100)
+follows its closed-form spin-down, saturated and not.
 """
 import numpy as np
 import pytest
@@ -280,3 +284,105 @@ def test_constant_time_lag_spin_relaxation(oracle, spin_over_n):
     np.testing.assert_allclose(oracle_spin, integrated_spin, rtol=0, atol=1e-7 * np.linalg.norm(integrated_spin))
     # the orbit: same radius (circular) and same phase to the step's error
     np.testing.assert_allclose(r1, sol.y[:3, -1], rtol=0, atol=1e-7 * a)
+
+
+@pytest.mark.parametrize("implementation", ["Anderson1975", "Newhall1983"])
+@pytest.mark.parametrize("a,e,M", [(0.05, 0.1, 1.0), (0.03, 0.3, 1.0), (0.05, 0.1, 0.5)])
+def test_other_general_relativity_implementations_give_einsteins_advance(oracle, implementation, a, e, M):
+    """The reference's two other GR implementations are first post-Newtonian
+    (general_relativity.rs:299-456: Anderson et al. 1975 for the host's mass
+    alone, Newhall et al. 1983 for every pair); for a planet of negligible
+    mass both advance the periapsis by Einstein's 6 pi G M / (c^2 a (1 - e^2))
+    per orbit, whatever their gauge (measured: 1e-5 to 2.4e-5 of it over 300
+    orbits)."""
+    m = gen.M_EARTH
+    mu = gen.G * (M + m)
+    period = 2 * np.pi * np.sqrt(a ** 3 / mu)
+    orbits, steps_per_orbit = 300, 100
+    p, v = gen.cartesian_from_elements(mu, np.array([a * (1 - e)]), np.array([e]), np.array([0.0]),
+                                       np.array([0.7]), np.array([0.0]), np.array([0.4]))
+    pos = np.zeros((1, 2, 3)); vel = np.zeros((1, 2, 3))
+    pos[0, 1], vel[0, 1] = p[0], v[0]
+    zeros = np.zeros((1, 2))
+    roles = np.array([[abi.ROLE_GR_CENTRAL, abi.ROLE_GR_ORBITING]], dtype=np.uint32)
+    spec = gen.build_spec([2], np.array([[M, m]]), np.array([[gen.R_SUN, gen.R_EARTH]]),
+                          np.array([[0.07, 0.33]]), pos, vel, np.zeros((1, 2, 3)), roles, zeros, np.ones((1, 2)),
+                          zeros, zeros, period / steps_per_orbit, abi.COORD_DEMOCRATIC_HELIOCENTRIC,
+                          effects=abi.EFFECT_GENERAL_RELATIVITY)
+    spec.system["gr_implementation"][:] = abi.GR_IMPLEMENTATIONS[implementation]
+    out = oracle.run(spec, orbits * steps_per_orbit, threads=1)
+    # (the Kepler solver's bisection fallback, whfast.rs:776-808, may be
+    # reported; it is not an error)
+    assert out["status"][0] & abi.STATUS_FATAL == 0
+    assert out["status"][0] & abi.STATUS_MIDPOINT_NOT_CONVERGED == 0
+    t_end = out["current_time"][0]
+    x0, v0 = spec.body["position"], spec.body["velocity"]
+    x1, v1 = out["position"], out["velocity"]
+    oracle_advance = advance(x0[1] - x0[0], v0[1] - v0[0], x1[1] - x1[0], v1[1] - v1[0], mu)
+    einstein = t_end / period * 6 * np.pi * mu / (C2 * a * (1 - e ** 2))
+    print(f"{implementation} a={a} e={e} M={M}: oracle / Einstein - 1 = {oracle_advance / einstein - 1:.2e}")
+    assert abs(oracle_advance / einstein - 1) < 2e-4
+
+
+def wind_spin(omega0, wsat, rate, t):
+    """Spin rate under the reference's wind torque dL/dt = -k spin
+    min(|spin|, w_sat)^2 sqrt(R / R_sun / M) (wind.rs:71-100), with rate =
+    k sqrt(R / R_sun / M) / I: exponential decay at rate w_sat^2 while
+    saturated, then d(Omega^-2)/dt = 2 rate."""
+    if omega0 < wsat:
+        return omega0 / np.sqrt(1 + 2 * rate * omega0 ** 2 * t)
+    t1 = np.log(omega0 / wsat) / (rate * wsat ** 2)
+    if t <= t1:
+        return omega0 * np.exp(-rate * wsat ** 2 * t)
+    return wsat / np.sqrt(1 + 2 * rate * wsat ** 2 * (t - t1))
+
+
+@pytest.mark.parametrize("omega0", [1.0, 3.0])
+def test_wind_spin_down(oracle, omega0):
+    """Magnetic braking of a star's spin, below and across the saturation
+    rate, against its closed form.  The spin is integrated only when tides,
+    flattening, evolution or Kidder GR are on (whfast.rs:274-275), so the
+    system also has tides, with zero dissipation and Love numbers (no tidal
+    force or torque); with the wind alone the spin does not move."""
+    M, R, rg2, m, a, wsat = 0.8, 0.9 * gen.R_SUN, 0.07, gen.M_EARTH, 0.1, 2.0
+    mu = gen.G * (M + m)
+    period = 2 * np.pi * np.sqrt(a ** 3 / mu)
+    inertia = rg2 * M * R ** 2
+    k = inertia / 2e3          # a spin-down time of order 1e3 days at 1 rad/day
+    rate = k * np.sqrt(R / gen.R_SUN / M) / inertia
+    p, v = gen.cartesian_from_elements(mu, np.array([a]), np.array([0.0]), np.array([0.0]),
+                                       np.array([0.3]), np.array([0.0]), np.array([0.4]))
+    pos = np.zeros((1, 2, 3)); vel = np.zeros((1, 2, 3))
+    pos[0, 1], vel[0, 1] = p[0], v[0]
+    axis = np.array([0.6, 0.0, 0.8])
+    spin = np.zeros((1, 2, 3))
+    spin[0, 0] = omega0 * axis
+    spin[0, 1, 2] = 1.0
+    roles = np.array([[abi.ROLE_WIND | abi.ROLE_TIDES_CENTRAL, abi.ROLE_TIDES_ORBITING]], dtype=np.uint32)
+    zeros = np.zeros((1, 2))
+    steps = int(round(3e3 / (period / 20)))
+    result = {}
+    for effects in (abi.EFFECT_WIND, abi.EFFECT_WIND | abi.EFFECT_TIDES):
+        spec = gen.build_spec([2], np.array([[M, m]]), np.array([[R, gen.R_EARTH]]), np.array([[rg2, 0.33]]),
+                              pos, vel, spin, roles, zeros, np.ones((1, 2)), zeros, zeros, period / 20,
+                              abi.COORD_DEMOCRATIC_HELIOCENTRIC, effects=effects)
+        spec.body["wind_k_factor"] = np.array([k, 0.0])
+        spec.body["wind_rotation_saturation"] = np.array([wsat, 0.0])
+        out = oracle.run(spec, steps, threads=1)
+        assert out["status"][0] & abi.STATUS_FATAL == 0
+        result[effects] = out
+    alone = result[abi.EFFECT_WIND]
+    np.testing.assert_array_equal(alone["angular_momentum"][0], inertia * spin[0, 0])
+    out = result[abi.EFFECT_WIND | abi.EFFECT_TIDES]
+    L = out["angular_momentum"][0]
+    t = out["current_time"][0]
+    omega = np.linalg.norm(L) / inertia
+    expected = wind_spin(omega0, wsat, rate, t)
+    print(f"omega0={omega0}: {omega:.12f} rad/day after {t:.0f} days, closed form {expected:.12f}")
+    # the spin keeps its direction and follows the closed form (the step's
+    # second-order error, and the step that crosses the saturation rate:
+    # measured 1e-9 and 7e-9)
+    assert np.linalg.norm(np.cross(L / np.linalg.norm(L), axis)) < 1e-14
+    assert abs(omega / expected - 1) < 1e-7
+    # the planet's spin has no torque
+    np.testing.assert_allclose(out["angular_momentum"][1] / (0.33 * m * gen.R_EARTH ** 2), spin[0, 1], rtol=1e-14)
