@@ -1,0 +1,65 @@
+"""The closed-form systems on the device: each must equal the oracle's run bit
+for bit over its full length (3e3 to 1e5 steps), so the device's apsidal
+advances under Kidder, Anderson and Newhall GR and under flattening, its
+constant-time-lag spin relaxation and its wind braking are the ones that
+tests/test_oracle_closed_forms.py checks against their closed forms.  These
+single-system runs also cover effect sets the benchmark ensembles do not
+have on their own: GR alone in each implementation, flattening alone, tides
+alone, the wind alone (whose spin is not integrated, whfast.rs:274-275) and
+the wind crossing its saturation rate."""
+import numpy as np
+import pytest
+
+from posidonius_amd import abi
+from tests import closed_form_cases as cases
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("position", "velocity", "angular_momentum", "spin", "velocity_error", "angular_momentum_error")
+
+
+def run_both(oracle, spec, steps):
+    from posidonius_amd.ensemble import Ensemble
+    print(f"{steps} steps", flush=True)
+    with Ensemble(spec, mapping="packed", steps_per_launch=1024, use_graph=True) as ens:
+        ens.step(steps)
+        out = ens.get_state()
+    ref = oracle.run(spec, steps)
+    for k in FIELDS:
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+    np.testing.assert_array_equal(out["status"], ref["status"])
+    np.testing.assert_array_equal(out["current_time"], ref["current_time"])
+    assert out["current_iteration"][0] == steps
+    assert out["status"][0] & abi.STATUS_FATAL == 0
+    return out
+
+
+@pytest.mark.parametrize("a,e,M", [(0.05, 0.1, 1.0), (0.03, 0.3, 1.0), (0.05, 0.1, 0.5)])
+def test_kidder_case(oracle, a, e, M):
+    run_both(oracle, *cases.kidder_case(a, e, M)[:2])
+
+
+@pytest.mark.parametrize("implementation", ["Anderson1975", "Newhall1983"])
+@pytest.mark.parametrize("a,e,M", [(0.05, 0.1, 1.0), (0.03, 0.3, 1.0), (0.05, 0.1, 0.5)])
+def test_other_gr_case(oracle, implementation, a, e, M):
+    run_both(oracle, *cases.other_gr_case(implementation, a, e, M)[:2])
+
+
+@pytest.mark.parametrize("a,e", [(0.03, 0.2), (0.05, 0.1)])
+def test_flattening_case(oracle, a, e):
+    run_both(oracle, *cases.flattening_case(a, e)[:2])
+
+
+@pytest.mark.parametrize("spin_over_n", [3.0, 0.2])
+def test_constant_time_lag_case(oracle, spin_over_n):
+    run_both(oracle, *cases.ctl_case(spin_over_n)[:2])
+
+
+@pytest.mark.parametrize("effects", ["wind", "wind+tides"])
+@pytest.mark.parametrize("omega0", [1.0, 3.0])
+def test_wind_case(oracle, omega0, effects):
+    mask = abi.EFFECT_WIND | (abi.EFFECT_TIDES if effects == "wind+tides" else 0)
+    spec, steps, P = cases.wind_case(omega0, mask)
+    out = run_both(oracle, spec, steps)
+    if effects == "wind":   # the spin is not integrated
+        np.testing.assert_array_equal(out["angular_momentum"][0], P["inertia"] * P["spin"][0, 0])

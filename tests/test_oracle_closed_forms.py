@@ -40,6 +40,8 @@ from scipy.integrate import solve_ivp
 
 from posidonius_amd import abi, generators as gen
 
+from tests import closed_form_cases as cases
+
 AU = 1.49597870700e11
 DAY = 86400.0
 C2 = (2.99792458e8 / AU * DAY) ** 2   # squared speed of light in AU^2/day^2 (src/constants.rs)
@@ -94,26 +96,9 @@ def integrate(r0, v0, t_end, M, m, second_order):
 
 @pytest.mark.parametrize("a,e,M", [(0.05, 0.1, 1.0), (0.03, 0.3, 1.0), (0.05, 0.1, 0.5)])
 def test_kidder_apsidal_advance(oracle, a, e, M):
-    m = gen.M_EARTH
-    mu = gen.G * (M + m)
-    period = 2 * np.pi * np.sqrt(a ** 3 / mu)
-    orbits, steps_per_orbit = 300, 100
-    p, v = gen.cartesian_from_elements(mu, np.array([a * (1 - e)]), np.array([e]), np.array([0.0]),
-                                       np.array([0.7]), np.array([0.0]), np.array([0.4]))
-    pos = np.zeros((1, 2, 3)); vel = np.zeros((1, 2, 3))
-    pos[0, 1], vel[0, 1] = p[0], v[0]
-    zeros = np.zeros((1, 2))
-    roles = np.array([[abi.ROLE_GR_CENTRAL, abi.ROLE_GR_ORBITING]], dtype=np.uint32)
-    # spins along the orbit normal, slow enough that their spin-orbit terms
-    # are below 1e-8 of the advance (with every spin zero the midpoint's
-    # convergence test divides zero by zero, as the reference's does)
-    spin = np.zeros((1, 2, 3))
-    spin[0, :, 2] = 2 * np.pi / 1e6
-    spec = gen.build_spec([2], np.array([[M, m]]), np.array([[gen.R_SUN, gen.R_EARTH]]),
-                          np.array([[0.07, 0.33]]), pos, vel, spin, roles, zeros, np.ones((1, 2)),
-                          zeros, zeros, period / steps_per_orbit, abi.COORD_DEMOCRATIC_HELIOCENTRIC,
-                          effects=abi.EFFECT_GENERAL_RELATIVITY)
-    out = oracle.run(spec, orbits * steps_per_orbit, threads=1)
+    spec, steps, P = cases.kidder_case(a, e, M)
+    m, mu, period = P["m"], P["mu"], P["period"]
+    out = oracle.run(spec, steps, threads=1)
     assert out["status"][0] == 0
     t_end = out["current_time"][0]
 
@@ -158,28 +143,12 @@ def test_rotational_flattening_apsidal_advance(oracle, a, e):
     J2 potential), which advances the periapsis at (3/2) n J2 (R / p)^2.  The
     planet's own fluid Love number is zero, the spins are aligned with the
     orbit normal (no torque, so they stay constant), only flattening is on."""
-    M, m = 1.0, gen.M_EARTH
-    R, k2f, omega = gen.R_SUN, 0.03, 2 * np.pi / 2.0
-    mu = gen.G * (M + m)
-    n = np.sqrt(mu / a ** 3)
-    period = 2 * np.pi / n
-    orbits, steps_per_orbit = 1000, 100
-    p, v = gen.cartesian_from_elements(mu, np.array([a * (1 - e)]), np.array([e]), np.array([0.0]),
-                                       np.array([1.1]), np.array([0.0]), np.array([2.0]))
-    pos = np.zeros((1, 2, 3)); vel = np.zeros((1, 2, 3))
-    pos[0, 1], vel[0, 1] = p[0], v[0]
-    roles = np.array([[abi.ROLE_FLAT_CENTRAL, abi.ROLE_FLAT_ORBITING]], dtype=np.uint32)
-    spin = np.zeros((1, 2, 3))
-    spin[0, 0, 2], spin[0, 1, 2] = omega, 2 * np.pi / 1.0
-    zeros = np.zeros((1, 2))
-    spec = gen.build_spec([2], np.array([[M, m]]), np.array([[R, gen.R_EARTH]]), np.array([[0.07, 0.33]]),
-                          pos, vel, spin, roles, zeros, np.ones((1, 2)), zeros, np.array([[k2f, 0.0]]),
-                          period / steps_per_orbit, abi.COORD_DEMOCRATIC_HELIOCENTRIC,
-                          effects=abi.EFFECT_ROTATIONAL_FLATTENING)
-    out = oracle.run(spec, orbits * steps_per_orbit, threads=1)
+    spec, steps, P = cases.flattening_case(a, e)
+    M, m, mu, n, period, R, k2f, omega = (P[k] for k in ("M", "m", "mu", "n", "period", "R", "k2f", "omega"))
+    out = oracle.run(spec, steps, threads=1)
     assert out["status"][0] == 0
     t_end = out["current_time"][0]
-    np.testing.assert_allclose(out["spin"][0], spin[0, 0], rtol=1e-12)
+    np.testing.assert_allclose(out["spin"][0], spec.body["spin"][0], rtol=1e-12)
 
     x0, v0 = spec.body["position"], spec.body["velocity"]
     r0, u0 = x0[1] - x0[0], v0[1] - v0[0]
@@ -221,28 +190,9 @@ def test_constant_time_lag_spin_relaxation(oracle, spin_over_n):
     momentum is 1e5 times the spin's, so a and n stay fixed to 1e-5).  An
     independent integration of the orbit and the spin under Hut's force
     follows the oracle."""
-    M, m, R, rg2, a = 0.08, gen.M_EARTH, gen.R_EARTH, 0.33, 0.01
-    k2dt = 10 * 2.465278e-3
-    mu = gen.G * (M + m)
-    n = np.sqrt(mu / a ** 3)
-    period = 2 * np.pi / n
-    tau = rg2 * m * a ** 6 / (3 * gen.G * M ** 2 * k2dt * R ** 3)
-    steps_per_orbit = 50
-    steps = int(round(1.5 * tau / (period / steps_per_orbit)))
-    p, v = gen.cartesian_from_elements(mu, np.array([a]), np.array([0.0]), np.array([0.0]),
-                                       np.array([0.6]), np.array([0.0]), np.array([1.3]))
-    pos = np.zeros((1, 2, 3)); vel = np.zeros((1, 2, 3))
-    pos[0, 1], vel[0, 1] = p[0], v[0]
-    normal = np.cross(p[0], v[0]); normal /= np.linalg.norm(normal)
-    spin = np.zeros((1, 2, 3))
-    spin[0, 0] = 2 * np.pi / 30.0 * normal
-    spin[0, 1] = spin_over_n * n * normal
-    roles = np.array([[abi.ROLE_TIDES_CENTRAL, abi.ROLE_TIDES_ORBITING]], dtype=np.uint32)
-    sigma = np.array([[0.0, 2 * gen.G * k2dt / (3 * R ** 5)]])
-    zeros = np.zeros((1, 2))
-    spec = gen.build_spec([2], np.array([[M, m]]), np.array([[0.1 * gen.R_SUN, R]]), np.array([[0.2, rg2]]),
-                          pos, vel, spin, roles, sigma, np.ones((1, 2)), zeros, zeros,
-                          period / steps_per_orbit, abi.COORD_DEMOCRATIC_HELIOCENTRIC, effects=abi.EFFECT_TIDES)
+    spec, steps, P = cases.ctl_case(spin_over_n)
+    M, m, R, rg2, a, k2dt, mu, n, tau, normal, spin = (P[k] for k in ("M", "m", "R", "rg2", "a", "k2dt", "mu", "n",
+                                                                      "tau", "normal", "spin"))
     out = oracle.run(spec, steps, threads=1)
     assert out["status"][0] == 0
     t_end = out["current_time"][0]
@@ -295,22 +245,9 @@ def test_other_general_relativity_implementations_give_einsteins_advance(oracle,
     mass both advance the periapsis by Einstein's 6 pi G M / (c^2 a (1 - e^2))
     per orbit, whatever their gauge (measured: 1e-5 to 2.4e-5 of it over 300
     orbits)."""
-    m = gen.M_EARTH
-    mu = gen.G * (M + m)
-    period = 2 * np.pi * np.sqrt(a ** 3 / mu)
-    orbits, steps_per_orbit = 300, 100
-    p, v = gen.cartesian_from_elements(mu, np.array([a * (1 - e)]), np.array([e]), np.array([0.0]),
-                                       np.array([0.7]), np.array([0.0]), np.array([0.4]))
-    pos = np.zeros((1, 2, 3)); vel = np.zeros((1, 2, 3))
-    pos[0, 1], vel[0, 1] = p[0], v[0]
-    zeros = np.zeros((1, 2))
-    roles = np.array([[abi.ROLE_GR_CENTRAL, abi.ROLE_GR_ORBITING]], dtype=np.uint32)
-    spec = gen.build_spec([2], np.array([[M, m]]), np.array([[gen.R_SUN, gen.R_EARTH]]),
-                          np.array([[0.07, 0.33]]), pos, vel, np.zeros((1, 2, 3)), roles, zeros, np.ones((1, 2)),
-                          zeros, zeros, period / steps_per_orbit, abi.COORD_DEMOCRATIC_HELIOCENTRIC,
-                          effects=abi.EFFECT_GENERAL_RELATIVITY)
-    spec.system["gr_implementation"][:] = abi.GR_IMPLEMENTATIONS[implementation]
-    out = oracle.run(spec, orbits * steps_per_orbit, threads=1)
+    spec, steps, P = cases.other_gr_case(implementation, a, e, M)
+    mu, period = P["mu"], P["period"]
+    out = oracle.run(spec, steps, threads=1)
     # (the Kepler solver's bisection fallback, whfast.rs:776-808, may be
     # reported; it is not an error)
     assert out["status"][0] & abi.STATUS_FATAL == 0
@@ -344,33 +281,13 @@ def test_wind_spin_down(oracle, omega0):
     flattening, evolution or Kidder GR are on (whfast.rs:274-275), so the
     system also has tides, with zero dissipation and Love numbers (no tidal
     force or torque); with the wind alone the spin does not move."""
-    M, R, rg2, m, a, wsat = 0.8, 0.9 * gen.R_SUN, 0.07, gen.M_EARTH, 0.1, 2.0
-    mu = gen.G * (M + m)
-    period = 2 * np.pi * np.sqrt(a ** 3 / mu)
-    inertia = rg2 * M * R ** 2
-    k = inertia / 2e3          # a spin-down time of order 1e3 days at 1 rad/day
-    rate = k * np.sqrt(R / gen.R_SUN / M) / inertia
-    p, v = gen.cartesian_from_elements(mu, np.array([a]), np.array([0.0]), np.array([0.0]),
-                                       np.array([0.3]), np.array([0.0]), np.array([0.4]))
-    pos = np.zeros((1, 2, 3)); vel = np.zeros((1, 2, 3))
-    pos[0, 1], vel[0, 1] = p[0], v[0]
-    axis = np.array([0.6, 0.0, 0.8])
-    spin = np.zeros((1, 2, 3))
-    spin[0, 0] = omega0 * axis
-    spin[0, 1, 2] = 1.0
-    roles = np.array([[abi.ROLE_WIND | abi.ROLE_TIDES_CENTRAL, abi.ROLE_TIDES_ORBITING]], dtype=np.uint32)
-    zeros = np.zeros((1, 2))
-    steps = int(round(3e3 / (period / 20)))
     result = {}
     for effects in (abi.EFFECT_WIND, abi.EFFECT_WIND | abi.EFFECT_TIDES):
-        spec = gen.build_spec([2], np.array([[M, m]]), np.array([[R, gen.R_EARTH]]), np.array([[rg2, 0.33]]),
-                              pos, vel, spin, roles, zeros, np.ones((1, 2)), zeros, zeros, period / 20,
-                              abi.COORD_DEMOCRATIC_HELIOCENTRIC, effects=effects)
-        spec.body["wind_k_factor"] = np.array([k, 0.0])
-        spec.body["wind_rotation_saturation"] = np.array([wsat, 0.0])
+        spec, steps, P = cases.wind_case(omega0, effects)
         out = oracle.run(spec, steps, threads=1)
         assert out["status"][0] & abi.STATUS_FATAL == 0
         result[effects] = out
+    m, inertia, wsat, rate, axis, spin = (P[k] for k in ("m", "inertia", "wsat", "rate", "axis", "spin"))
     alone = result[abi.EFFECT_WIND]
     np.testing.assert_array_equal(alone["angular_momentum"][0], inertia * spin[0, 0])
     out = result[abi.EFFECT_WIND | abi.EFFECT_TIDES]
