@@ -1,7 +1,8 @@
 """Star + planet systems whose orbit or spin evolution has a closed form,
 one builder per effect.  Each returns (spec, steps, params): the ensemble of
 one system, the number of steps to run it and the quantities its closed form
-needs.  tests/test_oracle_closed_forms.py checks the oracle's runs of them
+needs.  `coordinates` is the integrator's coordinate type (democratic
+heliocentric by default).  tests/test_oracle_closed_forms.py checks the oracle's runs of them
 against the closed forms and independent integrations; the device's runs
 must equal the oracle's bit for bit (tests/test_gpu_closed_forms.py)."""
 import numpy as np
@@ -19,7 +20,7 @@ def _orbit(mu, a, e, varpi, node, mean_anomaly):
     return pos, vel
 
 
-def kidder_case(a, e, M):
+def kidder_case(a, e, M, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
     """Only Kidder GR on; 300 orbits of 100 steps.  The spins lie along z,
     slow enough that their spin-orbit terms are below 1e-8 of the advance
     (with every spin zero the midpoint's convergence test divides zero by
@@ -35,12 +36,12 @@ def kidder_case(a, e, M):
     spin[0, :, 2] = 2 * np.pi / 1e6
     spec = gen.build_spec([2], np.array([[M, m]]), np.array([[gen.R_SUN, gen.R_EARTH]]),
                           np.array([[0.07, 0.33]]), pos, vel, spin, roles, zeros, np.ones((1, 2)),
-                          zeros, zeros, period / steps_per_orbit, abi.COORD_DEMOCRATIC_HELIOCENTRIC,
+                          zeros, zeros, period / steps_per_orbit, coordinates,
                           effects=abi.EFFECT_GENERAL_RELATIVITY)
     return spec, orbits * steps_per_orbit, dict(M=M, m=m, mu=mu, period=period)
 
 
-def other_gr_case(implementation, a, e, M):
+def other_gr_case(implementation, a, e, M, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
     """Only GR on, with the Anderson 1975 or Newhall 1983 implementation (no
     spin integration); 300 orbits of 100 steps."""
     m = gen.M_EARTH
@@ -52,13 +53,13 @@ def other_gr_case(implementation, a, e, M):
     roles = np.array([[abi.ROLE_GR_CENTRAL, abi.ROLE_GR_ORBITING]], dtype=np.uint32)
     spec = gen.build_spec([2], np.array([[M, m]]), np.array([[gen.R_SUN, gen.R_EARTH]]),
                           np.array([[0.07, 0.33]]), pos, vel, np.zeros((1, 2, 3)), roles, zeros, np.ones((1, 2)),
-                          zeros, zeros, period / steps_per_orbit, abi.COORD_DEMOCRATIC_HELIOCENTRIC,
+                          zeros, zeros, period / steps_per_orbit, coordinates,
                           effects=abi.EFFECT_GENERAL_RELATIVITY)
     spec.system["gr_implementation"][:] = abi.GR_IMPLEMENTATIONS[implementation]
     return spec, orbits * steps_per_orbit, dict(M=M, m=m, mu=mu, period=period)
 
 
-def flattening_case(a, e):
+def flattening_case(a, e, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
     """Only rotational flattening on, the star spinning along the normal of
     an orbit in its equatorial plane (the z axis), the planet's fluid Love
     number zero;
@@ -79,12 +80,12 @@ def flattening_case(a, e):
     zeros = np.zeros((1, 2))
     spec = gen.build_spec([2], np.array([[M, m]]), np.array([[R, gen.R_EARTH]]), np.array([[0.07, 0.33]]),
                           pos, vel, spin, roles, zeros, np.ones((1, 2)), zeros, np.array([[k2f, 0.0]]),
-                          period / steps_per_orbit, abi.COORD_DEMOCRATIC_HELIOCENTRIC,
+                          period / steps_per_orbit, coordinates,
                           effects=abi.EFFECT_ROTATIONAL_FLATTENING)
     return spec, orbits * steps_per_orbit, dict(M=M, m=m, mu=mu, n=n, period=period, R=R, k2f=k2f, omega=omega)
 
 
-def ctl_case(spin_over_n):
+def ctl_case(spin_over_n, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
     """Only tides on, dissipating in the planet alone (sigma = 2 G k2 dt /
     (3 R^5), as the reference's example cases set it), a circular orbit and
     the spins along its normal; 1.5 spin-relaxation times at 50 steps per
@@ -108,12 +109,12 @@ def ctl_case(spin_over_n):
     zeros = np.zeros((1, 2))
     spec = gen.build_spec([2], np.array([[M, m]]), np.array([[0.1 * gen.R_SUN, R]]), np.array([[0.2, rg2]]),
                           pos, vel, spin, roles, sigma, np.ones((1, 2)), zeros, zeros,
-                          period / steps_per_orbit, abi.COORD_DEMOCRATIC_HELIOCENTRIC, effects=abi.EFFECT_TIDES)
+                          period / steps_per_orbit, coordinates, effects=abi.EFFECT_TIDES)
     return spec, steps, dict(M=M, m=m, R=R, rg2=rg2, a=a, k2dt=k2dt, mu=mu, n=n, period=period, tau=tau,
                              normal=normal, spin=spin)
 
 
-def wind_case(omega0, effects):
+def wind_case(omega0, effects, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
     """A 0.8 solar-mass star of 0.9 solar radii braked by its wind (k giving a
     spin-down time of order 1e3 days at 1 rad/day, saturation at 2 rad/day),
     with a planet at 0.1 AU; 3,000 days at 20 steps per orbit.  effects is
@@ -133,7 +134,7 @@ def wind_case(omega0, effects):
     zeros = np.zeros((1, 2))
     spec = gen.build_spec([2], np.array([[M, m]]), np.array([[R, gen.R_EARTH]]), np.array([[rg2, 0.33]]),
                           pos, vel, spin, roles, zeros, np.ones((1, 2)), zeros, zeros, period / 20,
-                          abi.COORD_DEMOCRATIC_HELIOCENTRIC, effects=effects)
+                          coordinates, effects=effects)
     spec.body["wind_k_factor"] = np.array([k, 0.0])
     spec.body["wind_rotation_saturation"] = np.array([wsat, 0.0])
     steps = int(round(3e3 / (period / 20)))

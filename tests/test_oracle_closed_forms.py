@@ -303,3 +303,69 @@ def test_wind_spin_down(oracle, omega0):
     assert abs(omega / expected - 1) < 1e-7
     # the planet's spin has no torque
     np.testing.assert_allclose(out["angular_momentum"][1] / (0.33 * m * gen.R_EARTH ** 2), spin[0, 1], rtol=1e-14)
+
+
+def _ok(status):
+    # (the Kepler solver's bisection fallback, whfast.rs:776-808, may be
+    # reported; it is not an error)
+    return status & abi.STATUS_FATAL == 0 and status & abi.STATUS_MIDPOINT_NOT_CONVERGED == 0
+
+
+@pytest.mark.parametrize("coordinates", ["Jacobi", "WHDS"])
+def test_closed_forms_in_jacobi_and_whds_coordinates(oracle, coordinates):
+    """The same closed-form systems integrated in Jacobi and WHDS coordinates
+    (whfast.rs:881-1155): the physics does not depend on the splitting, so
+    each meets the bound it meets in democratic heliocentric coordinates
+    (one case per effect; measured: the same agreement to 1e-8)."""
+    c = abi.COORDINATES[coordinates]
+    dh = abi.COORD_DEMOCRATIC_HELIOCENTRIC
+
+    def adv(spec, out, mu):
+        x0, v0 = spec.body["position"], spec.body["velocity"]
+        x1, v1 = out["position"], out["velocity"]
+        return advance(x0[1] - x0[0], v0[1] - v0[0], x1[1] - x1[0], v1[1] - v1[0], mu)
+
+    a, e = 0.03, 0.3
+    for implementation in ("Anderson1975", "Newhall1983"):
+        spec, steps, P = cases.other_gr_case(implementation, a, e, 1.0, c)
+        out = oracle.run(spec, steps)
+        assert _ok(out["status"][0])
+        einstein = out["current_time"][0] / P["period"] * 6 * np.pi * P["mu"] / (C2 * a * (1 - e ** 2))
+        assert abs(adv(spec, out, P["mu"]) / einstein - 1) < 2e-4, implementation
+    # Kidder: the democratic heliocentric run's advance is the one the
+    # independent integration of the reference's equations pins
+    advances = []
+    for coords in (c, dh):
+        spec, steps, P = cases.kidder_case(a, e, 1.0, coords)
+        out = oracle.run(spec, steps)
+        assert _ok(out["status"][0])
+        advances.append(adv(spec, out, P["mu"]))
+    assert abs(advances[0] / advances[1] - 1) < 1e-6, advances
+
+    a, e = 0.03, 0.2
+    spec, steps, P = cases.flattening_case(a, e, c)
+    out = oracle.run(spec, steps)
+    assert _ok(out["status"][0])
+    J2 = P["k2f"] * P["omega"] ** 2 * P["R"] ** 3 / (3 * gen.G * P["M"])
+    closed = 1.5 * P["n"] * J2 * (P["R"] / (a * (1 - e ** 2))) ** 2 * out["current_time"][0]
+    assert abs(adv(spec, out, P["mu"]) / closed - 1) < 2e-3
+
+    for spin_over_n in (3.0, 0.2):
+        spec, steps, P = cases.ctl_case(spin_over_n, c)
+        out = oracle.run(spec, steps)
+        assert _ok(out["status"][0])
+        omega = (out["angular_momentum"][1] / (P["rg2"] * P["m"] * P["R"] ** 2)) @ P["normal"]
+        r1 = out["position"][1] - out["position"][0]
+        u1 = out["velocity"][1] - out["velocity"][0]
+        mu = P["mu"]
+        n_end = np.sqrt(mu / (1 / (2 / np.linalg.norm(r1) - u1 @ u1 / mu)) ** 3)
+        rate = -np.log((omega - n_end) / ((spin_over_n - 1) * P["n"])) / out["current_time"][0]
+        assert abs(rate * P["tau"] - 1) < 1e-3, spin_over_n
+
+    for omega0 in (1.0, 3.0):
+        spec, steps, P = cases.wind_case(omega0, abi.EFFECT_WIND | abi.EFFECT_TIDES, c)
+        out = oracle.run(spec, steps)
+        assert _ok(out["status"][0])
+        omega = np.linalg.norm(out["angular_momentum"][0]) / P["inertia"]
+        expected = wind_spin(omega0, P["wsat"], P["rate"], out["current_time"][0])
+        assert abs(omega / expected - 1) < 1e-7, omega0
