@@ -85,3 +85,67 @@ def test_case_in_jacobi_and_whds_coordinates(oracle, case, coordinates):
     bounds tests/test_oracle_closed_forms.py checks on the oracle)."""
     spec, steps, _ = OTHER_COORDINATE_CASES[case](abi.COORDINATES[coordinates])
     run_both(oracle, spec, steps)
+
+
+def _adjoint(spec, steps, w_L):
+    from posidonius_amd.ensemble import Ensemble
+    print(f"adjoint over {steps} steps", flush=True)
+    with Ensemble(spec, mapping="packed", steps_per_launch=64) as ens:
+        ens.adjoint_record(steps, segment_steps=1000)
+        st = ens.get_state()
+        g = ens.adjoint(angular_momentum=w_L)
+    return st, g
+
+
+@pytest.mark.parametrize("spin_over_n", [3.0, 0.2])
+def test_adjoint_spin_relaxation_derivatives(oracle, spin_over_n):
+    """The device's discrete adjoint on the tidal spin relaxation: the
+    gradient of the final spin angular momentum along the orbit normal with
+    respect to the planet's dissipation factor and its initial spin equals
+    the oracle's tangent-linear derivatives to 1e-6 and the closed form to
+    1e-3."""
+    from tests.test_oracle_closed_forms import ctl_closed_form_derivatives
+    spec, steps, P = cases.ctl_case(spin_over_n)
+    nb = spec.n_bodies
+    w = np.zeros((nb, 3))
+    w[1] = P["normal"]
+    st, g = _adjoint(spec, steps, w)
+    ref = oracle.run(spec, steps)
+    np.testing.assert_array_equal(st["angular_momentum"], ref["angular_momentum"])
+    dirs = {"sigma": np.zeros((2, nb)), "L": np.zeros((2, nb, 3))}
+    dirs["sigma"][0, 1] = 1.0
+    dirs["L"][1, 1] = P["normal"]
+    _, tan = oracle.tangent(spec, steps, dirs)
+    jvp_sigma = tan["L"][0, 1] @ P["normal"]
+    jvp_L0 = tan["L"][1, 1] @ P["normal"]
+    vjp_sigma = g["dissipation_factor"][1]
+    vjp_L0 = g["angular_momentum"][1] @ P["normal"]
+    d_sigma, d_L0 = ctl_closed_form_derivatives(st, P)
+    print(f"d/dsigma: adjoint {vjp_sigma:.9e}, tangent {jvp_sigma:.9e}, closed form {d_sigma:.9e}; "
+          f"d/dL0: adjoint {vjp_L0:.9e}, tangent {jvp_L0:.9e}, closed form {d_L0:.9e}", flush=True)
+    assert abs(vjp_sigma / jvp_sigma - 1) < 1e-6
+    assert abs(vjp_L0 / jvp_L0 - 1) < 1e-6
+    assert abs(vjp_sigma / d_sigma - 1) < 1e-3
+    assert abs(vjp_L0 / d_L0 - 1) < 1e-3
+
+
+@pytest.mark.parametrize("omega0", [1.0, 3.0])
+def test_adjoint_wind_derivative(oracle, omega0):
+    """The device's discrete adjoint of the wind's spin-down: dOmega(t)/dOmega0
+    equals the oracle's tangent-linear derivative to 1e-6 and the closed form
+    to 1e-6 unsaturated, 1e-3 across the saturation rate."""
+    from tests.test_oracle_closed_forms import wind_closed_form_derivative
+    spec, steps, P = cases.wind_case(omega0, abi.EFFECT_WIND | abi.EFFECT_TIDES)
+    nb = spec.n_bodies
+    w = np.zeros((nb, 3))
+    w[0] = P["axis"]
+    st, g = _adjoint(spec, steps, w)
+    dirs = {"L": np.zeros((1, nb, 3))}
+    dirs["L"][0, 0] = P["axis"]
+    _, tan = oracle.tangent(spec, steps, dirs)
+    jvp = tan["L"][0, 0] @ P["axis"]
+    vjp = g["angular_momentum"][0] @ P["axis"]
+    closed = wind_closed_form_derivative(omega0, P, st["current_time"][0])
+    print(f"dOmega/dOmega0: adjoint {vjp:.12e}, tangent {jvp:.12e}, closed form {closed:.12e}", flush=True)
+    assert abs(vjp / jvp - 1) < 1e-6
+    assert abs(vjp / closed - 1) < (1e-6 if omega0 < P["wsat"] else 1e-3)
