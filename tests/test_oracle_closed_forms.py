@@ -427,3 +427,40 @@ def test_tangent_linear_wind_derivative(oracle, omega0):
     closed = wind_closed_form_derivative(omega0, P, st["current_time"][0])
     assert abs((tan["L"][0, 0] @ P["axis"]) / closed - 1) < (1e-6 if omega0 < P["wsat"] else 1e-3)
     assert np.linalg.norm(np.cross(tan["L"][0, 0], P["axis"])) < 1e-12
+
+
+def advance_gradient(spec, st, mu):
+    """The apsidal advance of the final state (from the initial one) and its
+    gradient with respect to the final relative position and velocity
+    (central differences of `advance`, steps 1e-6 of each vector's norm)."""
+    x0, v0 = spec.body["position"], spec.body["velocity"]
+    r0, u0 = x0[1] - x0[0], v0[1] - v0[0]
+    r1 = st["position"][1] - st["position"][0]
+    u1 = st["velocity"][1] - st["velocity"][0]
+    A = advance(r0, u0, r1, u1, mu)
+    gr, gv = np.zeros(3), np.zeros(3)
+    for q in range(3):
+        d = np.zeros(3)
+        d[q] = h = 1e-6 * np.linalg.norm(r1)
+        gr[q] = (advance(r0, u0, r1 + d, u1, mu) - advance(r0, u0, r1 - d, u1, mu)) / (2 * h)
+        d = np.zeros(3)
+        d[q] = h = 1e-6 * np.linalg.norm(u1)
+        gv[q] = (advance(r0, u0, r1, u1 + d, mu) - advance(r0, u0, r1, u1 - d, mu)) / (2 * h)
+    return A, gr, gv
+
+
+@pytest.mark.parametrize("a,e", [(0.03, 0.2), (0.05, 0.1)])
+def test_tangent_linear_flattening_advance_derivative(oracle, a, e):
+    """The J2 advance is linear in the star's fluid Love number (J2 = k2f
+    Omega^2 R^3 / (3 G M)), so its derivative with respect to k2f is the
+    advance over k2f, to first order in J2 (R/a)^2 ~ 1e-6: the oracle's
+    tangent-linear derivatives of the final state, contracted with the
+    advance's gradient, give it (measured: 1.5e-6)."""
+    spec, steps, P = cases.flattening_case(a, e)
+    nb = spec.n_bodies
+    dirs = {"k2f": np.zeros((1, nb))}
+    dirs["k2f"][0, 0] = 1.0
+    st, tan = oracle.tangent(spec, steps, dirs)
+    A, gr, gv = advance_gradient(spec, st, P["mu"])
+    dA = gr @ (tan["x"][0, 1] - tan["x"][0, 0]) + gv @ (tan["v"][0, 1] - tan["v"][0, 0])
+    assert abs(dA / (A / P["k2f"]) - 1) < 2e-5
