@@ -140,3 +140,25 @@ def wind_case(omega0, effects, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
     steps = int(round(3e3 / (period / 20)))
     return spec, steps, dict(M=M, R=R, m=m, inertia=inertia, k=k, wsat=wsat, axis=axis, spin=spin,
                              rate=k * np.sqrt(R / gen.R_SUN / M) / inertia)
+
+
+def tidal_love_case(M, m, R, a, e, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
+    """Only tides on with zero dissipation: the conservative tide the star
+    raises on the planet (Love number 0.3; the star's is zero), the spins
+    along the orbit normal (no torque without dissipation); 1,000 orbits of
+    100 steps."""
+    k2 = 0.3
+    mu = gen.G * (M + m)
+    n = np.sqrt(mu / a ** 3)
+    period = 2 * np.pi / n
+    orbits, steps_per_orbit = 1000, 100
+    pos, vel = _orbit(mu, a, e, 0.7, 0.0, 0.4)
+    spin = np.zeros((1, 2, 3))
+    spin[0, 0, 2] = 2 * np.pi / 30.0
+    spin[0, 1, 2] = n
+    roles = np.array([[abi.ROLE_TIDES_CENTRAL, abi.ROLE_TIDES_ORBITING]], dtype=np.uint32)
+    zeros = np.zeros((1, 2))
+    spec = gen.build_spec([2], np.array([[M, m]]), np.array([[gen.R_SUN, R]]), np.array([[0.07, 0.25]]),
+                          pos, vel, spin, roles, zeros, np.ones((1, 2)), np.array([[0.0, k2]]), zeros,
+                          period / steps_per_orbit, coordinates, effects=abi.EFFECT_TIDES)
+    return spec, orbits * steps_per_orbit, dict(M=M, m=m, R=R, a=a, e=e, k2=k2, mu=mu, n=n, period=period)

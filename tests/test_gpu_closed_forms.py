@@ -179,3 +179,30 @@ def test_adjoint_flattening_advance_derivative(oracle, a, e):
     print(f"dA/dk2f: adjoint {vjp:.12e}, tangent {jvp:.12e}, A/k2f {A / P['k2f']:.12e}", flush=True)
     assert abs(vjp / jvp - 1) < 1e-6
     assert abs(vjp / (A / P["k2f"]) - 1) < 2e-5
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_conservative_tide_case_and_adjoint(oracle, case):
+    """The conservative-tide system on the device, bitwise against the oracle,
+    and the device adjoint's gradient of its apsidal advance with respect to
+    the planet's Love number against the oracle's tangent-linear derivative
+    (1e-6); tests/test_oracle_closed_forms.py checks both against the closed
+    form."""
+    from tests.test_oracle_closed_forms import TIDAL_LOVE_CASES, advance_gradient
+    spec, steps, P = cases.tidal_love_case(*TIDAL_LOVE_CASES[case])
+    out = run_both(oracle, spec, steps)
+    nb = spec.n_bodies
+    A, gr, gv = advance_gradient(spec, out, P["mu"])
+    wx, wv = np.zeros((nb, 3)), np.zeros((nb, 3))
+    wx[1], wx[0], wv[1], wv[0] = gr, -gr, gv, -gv
+    from posidonius_amd.ensemble import Ensemble
+    with Ensemble(spec, mapping="packed", steps_per_launch=64) as ens:
+        ens.adjoint_record(steps, segment_steps=1000)
+        g = ens.adjoint(position=wx, velocity=wv)
+    dirs = {"k2": np.zeros((1, nb))}
+    dirs["k2"][0, 1] = 1.0
+    _, tan = oracle.tangent(spec, steps, dirs)
+    jvp = gr @ (tan["x"][0, 1] - tan["x"][0, 0]) + gv @ (tan["v"][0, 1] - tan["v"][0, 0])
+    vjp = g["love_number"][1]
+    print(f"dA/dk2: adjoint {vjp:.12e}, tangent {jvp:.12e}, A/k2 {A / P['k2']:.12e}", flush=True)
+    assert abs(vjp / jvp - 1) < 1e-6
