@@ -162,3 +162,27 @@ def tidal_love_case(M, m, R, a, e, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC
                           pos, vel, spin, roles, zeros, np.ones((1, 2)), np.array([[0.0, k2]]), zeros,
                           period / steps_per_orbit, coordinates, effects=abi.EFFECT_TIDES)
     return spec, orbits * steps_per_orbit, dict(M=M, m=m, R=R, a=a, e=e, k2=k2, mu=mu, n=n, period=period)
+
+
+def decay_case(k2dt, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
+    """Only tides on, dissipating in the star alone (sigma = 2 G k2 dt /
+    (3 R^5) with the given k2 dt, days; the Love numbers zero), an Earth-mass
+    planet on a circular orbit at 0.02 AU around a solar-mass star that
+    barely rotates (1e-6 of the mean motion, along the orbit normal);
+    5,000 steps of a hundredth of the period."""
+    M, m, Rs, a = 1.0, gen.M_EARTH, gen.R_SUN, 0.02
+    mu = gen.G * (M + m)
+    n = np.sqrt(mu / a ** 3)
+    period = 2 * np.pi / n
+    pos, vel = _orbit(mu, a, 0.0, 0.3, 0.0, 0.9)
+    spin = np.zeros((1, 2, 3))
+    spin[0, 0, 2] = 1e-6 * n
+    spin[0, 1, 2] = n
+    roles = np.array([[abi.ROLE_TIDES_CENTRAL, abi.ROLE_TIDES_ORBITING]], dtype=np.uint32)
+    zeros = np.zeros((1, 2))
+    sigma = 2 * gen.G * k2dt / (3 * Rs ** 5)
+    spec = gen.build_spec([2], np.array([[M, m]]), np.array([[Rs, gen.R_EARTH]]), np.array([[0.07, 0.33]]),
+                          pos, vel, spin, roles, np.array([[sigma, 0.0]]), np.ones((1, 2)), zeros, zeros,
+                          period / 100, coordinates, effects=abi.EFFECT_TIDES)
+    return spec, 5000, dict(M=M, m=m, Rs=Rs, a=a, k2dt=k2dt, sigma=sigma, mu=mu, n=n,
+                            inertia=0.07 * M * Rs ** 2)

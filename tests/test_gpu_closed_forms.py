@@ -206,3 +206,30 @@ def test_conservative_tide_case_and_adjoint(oracle, case):
     vjp = g["love_number"][1]
     print(f"dA/dk2: adjoint {vjp:.12e}, tangent {jvp:.12e}, A/k2 {A / P['k2']:.12e}", flush=True)
     assert abs(vjp / jvp - 1) < 1e-6
+
+
+@pytest.mark.parametrize("k2dt", [1e3, 3e3])
+def test_stellar_tide_decay_case_and_adjoint(oracle, k2dt):
+    """The orbital-decay system on the device, bitwise against the oracle, and
+    the device adjoint's gradient of the final semi-major axis with respect
+    to the star's dissipation factor against the oracle's tangent-linear
+    derivative (1e-6); tests/test_oracle_closed_forms.py checks both against
+    the closed form."""
+    from tests.test_oracle_closed_forms import semi_major_axis_and_gradient
+    spec, steps, P = cases.decay_case(k2dt)
+    out = run_both(oracle, spec, steps)
+    nb = spec.n_bodies
+    _, ga_r, ga_v = semi_major_axis_and_gradient(out, P["mu"])
+    wx, wv = np.zeros((nb, 3)), np.zeros((nb, 3))
+    wx[1], wx[0], wv[1], wv[0] = ga_r, -ga_r, ga_v, -ga_v
+    from posidonius_amd.ensemble import Ensemble
+    with Ensemble(spec, mapping="packed", steps_per_launch=64) as ens:
+        ens.adjoint_record(steps, segment_steps=1000)
+        g = ens.adjoint(position=wx, velocity=wv)
+    dirs = {"sigma": np.zeros((1, nb))}
+    dirs["sigma"][0, 0] = 1.0
+    _, tan = oracle.tangent(spec, steps, dirs)
+    jvp = ga_r @ (tan["x"][0, 1] - tan["x"][0, 0]) + ga_v @ (tan["v"][0, 1] - tan["v"][0, 0])
+    vjp = g["dissipation_factor"][0]
+    print(f"da/dsigma_star: adjoint {vjp:.12e}, tangent {jvp:.12e}", flush=True)
+    assert abs(vjp / jvp - 1) < 1e-6

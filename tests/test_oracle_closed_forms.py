@@ -502,3 +502,49 @@ def test_conservative_tide_apsidal_advance(oracle, M, m, R, a, e):
     # measured 1e-6 and 5e-5)
     dA = gr @ (tan["x"][0, 1] - tan["x"][0, 0]) + gv @ (tan["v"][0, 1] - tan["v"][0, 0])
     assert abs(dA / (A / P["k2"]) - 1) < 2e-4
+
+
+def semi_major_axis_and_gradient(st, mu):
+    """Osculating semi-major axis of the final relative orbit, 1 / (2/r -
+    v^2/mu), and its gradient with respect to the final relative position
+    and velocity."""
+    r1 = st["position"][1] - st["position"][0]
+    u1 = st["velocity"][1] - st["velocity"][0]
+    r = np.linalg.norm(r1)
+    a = 1 / (2 / r - u1 @ u1 / mu)
+    return a, 2 * a ** 2 * r1 / r ** 3, 2 * a ** 2 * u1 / mu
+
+
+@pytest.mark.parametrize("k2dt", [1e3, 3e3])
+def test_stellar_tide_orbital_decay(oracle, k2dt):
+    """The tide the planet raises on a star that barely rotates: the
+    tangential force 3 G m^2 k2 dt R^5 (Omega_star - n) / a^7 (Hut 1981, the
+    reference's constant_time_lag.rs:206-332 with the star's factor) shrinks
+    a circular orbit as da/dt = -6 G m k2 dt R^5 / a^7, so a^8 = a0^8 - 48 G
+    m k2 dt R^5 t (Omega_star / n stays below 1e-4).  The oracle's decay meets
+    it (measured 4e-6 and 5e-5 of the decay, the second with e grown to 6e-4),
+    the star's spin gains the orbital angular momentum lost, and the
+    tangent-linear derivative of the final a with respect to the star's
+    dissipation factor, -9 m R^10 t / a^7 (from a^8 = a0^8 - 72 m R^10 sigma
+    t), is met to the same order."""
+    spec, steps, P = cases.decay_case(k2dt)
+    nb = spec.n_bodies
+    dirs = {"sigma": np.zeros((1, nb))}
+    dirs["sigma"][0, 0] = 1.0
+    st, tan = oracle.tangent(spec, steps, dirs)
+    assert _ok(st["status"][0])
+    t = st["current_time"][0]
+    a0, mu, m, Rs = P["a"], P["mu"], P["m"], P["Rs"]
+    a1, ga_r, ga_v = semi_major_axis_and_gradient(st, mu)
+    closed = (a0 ** 8 - 48 * gen.G * m * k2dt * Rs ** 5 * t) ** 0.125
+    print(f"k2dt={k2dt}: a {a0} -> {a1:.12f}, closed form {closed:.12f}", flush=True)
+    assert abs((a1 - a0) / (closed - a0) - 1) < 2e-4
+    # angular momentum: the star's spin gains what the orbit loses (the
+    # orbit stays circular to e^2 ~ 4e-7)
+    mred = P["M"] * m / (P["M"] + m)
+    dL_orbit = mred * (np.sqrt(mu * a1) - np.sqrt(mu * a0))
+    dL_star = st["angular_momentum"][0][2] - spec.body["angular_momentum"][0][2]
+    assert abs(dL_star / -dL_orbit - 1) < 1e-5
+    # d a1 / d sigma_star
+    da = ga_r @ (tan["x"][0, 1] - tan["x"][0, 0]) + ga_v @ (tan["v"][0, 1] - tan["v"][0, 0])
+    assert abs(da / (-9 * m * Rs ** 10 * t / a1 ** 7) - 1) < 2e-4
