@@ -186,3 +186,25 @@ def decay_case(k2dt, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
                           period / 100, coordinates, effects=abi.EFFECT_TIDES)
     return spec, 5000, dict(M=M, m=m, Rs=Rs, a=a, k2dt=k2dt, sigma=sigma, mu=mu, n=n,
                             inertia=0.07 * M * Rs ** 2)
+
+
+def planet_flattening_case(a, e, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
+    """Only rotational flattening on, the planet's own bulge: an Earth-like
+    planet (fluid Love number 0.3, one-day spin along the normal of its
+    orbit, the z axis) around a 0.08 solar-mass star whose fluid Love number
+    is zero; 1,000 orbits of 100 steps."""
+    M, m, R = 0.08, gen.M_EARTH, gen.R_EARTH
+    k2f, omega = 0.3, 2 * np.pi / 1.0
+    mu = gen.G * (M + m)
+    n = np.sqrt(mu / a ** 3)
+    period = 2 * np.pi / n
+    pos, vel = _orbit(mu, a, e, 1.1, 0.0, 2.0)
+    roles = np.array([[abi.ROLE_FLAT_CENTRAL, abi.ROLE_FLAT_ORBITING]], dtype=np.uint32)
+    spin = np.zeros((1, 2, 3))
+    spin[0, 0, 2] = 2 * np.pi / 30.0
+    spin[0, 1, 2] = omega
+    zeros = np.zeros((1, 2))
+    spec = gen.build_spec([2], np.array([[M, m]]), np.array([[0.1 * gen.R_SUN, R]]), np.array([[0.2, 0.33]]),
+                          pos, vel, spin, roles, zeros, np.ones((1, 2)), zeros, np.array([[0.0, k2f]]),
+                          period / 100, coordinates, effects=abi.EFFECT_ROTATIONAL_FLATTENING)
+    return spec, 100 * 1000, dict(M=M, m=m, R=R, k2f=k2f, omega=omega, mu=mu, n=n, period=period)

@@ -233,3 +233,30 @@ def test_stellar_tide_decay_case_and_adjoint(oracle, k2dt):
     vjp = g["dissipation_factor"][0]
     print(f"da/dsigma_star: adjoint {vjp:.12e}, tangent {jvp:.12e}", flush=True)
     assert abs(vjp / jvp - 1) < 1e-6
+
+
+@pytest.mark.parametrize("a,e", [(0.01, 0.1), (0.015, 0.3)])
+def test_planet_flattening_case_and_adjoint(oracle, a, e):
+    """The planet's-bulge system on the device, bitwise against the oracle,
+    and the device adjoint's gradient of the apsidal advance with respect to
+    the planet's fluid Love number against the oracle's tangent-linear
+    derivative (1e-6); tests/test_oracle_closed_forms.py checks both against
+    the closed form."""
+    from tests.test_oracle_closed_forms import advance_gradient
+    spec, steps, P = cases.planet_flattening_case(a, e)
+    out = run_both(oracle, spec, steps)
+    nb = spec.n_bodies
+    _, gr, gv = advance_gradient(spec, out, P["mu"])
+    wx, wv = np.zeros((nb, 3)), np.zeros((nb, 3))
+    wx[1], wx[0], wv[1], wv[0] = gr, -gr, gv, -gv
+    from posidonius_amd.ensemble import Ensemble
+    with Ensemble(spec, mapping="packed", steps_per_launch=64) as ens:
+        ens.adjoint_record(steps, segment_steps=1000)
+        g = ens.adjoint(position=wx, velocity=wv)
+    dirs = {"k2f": np.zeros((1, nb))}
+    dirs["k2f"][0, 1] = 1.0
+    _, tan = oracle.tangent(spec, steps, dirs)
+    jvp = gr @ (tan["x"][0, 1] - tan["x"][0, 0]) + gv @ (tan["v"][0, 1] - tan["v"][0, 0])
+    vjp = g["fluid_love_number"][1]
+    print(f"dA/dk2f_planet: adjoint {vjp:.12e}, tangent {jvp:.12e}", flush=True)
+    assert abs(vjp / jvp - 1) < 1e-6
