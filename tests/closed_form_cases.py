@@ -208,3 +208,26 @@ def planet_flattening_case(a, e, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
                           pos, vel, spin, roles, zeros, np.ones((1, 2)), zeros, np.array([[0.0, k2f]]),
                           period / 100, coordinates, effects=abi.EFFECT_ROTATIONAL_FLATTENING)
     return spec, 100 * 1000, dict(M=M, m=m, R=R, k2f=k2f, omega=omega, mu=mu, n=n, period=period)
+
+
+def spin_orbit_case(a, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
+    """Only Kidder GR on, an Earth-mass planet on a circular orbit around a
+    solar-mass star spinning with a 10-day period about an axis 30 degrees
+    from the orbit normal (the z axis); the planet's spin slow (1e-6 rad/day)
+    along z; 1e5 steps of a hundredth of the period."""
+    M, m = 1.0, gen.M_EARTH
+    mu = gen.G * (M + m)
+    n = np.sqrt(mu / a ** 3)
+    period = 2 * np.pi / n
+    pos, vel = _orbit(mu, a, 0.0, 0.7, 0.0, 0.4)
+    tilt = np.radians(30.0)
+    axis = np.array([np.sin(tilt), 0.0, np.cos(tilt)])
+    spin = np.zeros((1, 2, 3))
+    spin[0, 0] = 2 * np.pi / 10.0 * axis
+    spin[0, 1, 2] = 2 * np.pi / 1e6
+    roles = np.array([[abi.ROLE_GR_CENTRAL, abi.ROLE_GR_ORBITING]], dtype=np.uint32)
+    zeros = np.zeros((1, 2))
+    spec = gen.build_spec([2], np.array([[M, m]]), np.array([[gen.R_SUN, gen.R_EARTH]]), np.array([[0.07, 0.33]]),
+                          pos, vel, spin, roles, zeros, np.ones((1, 2)), zeros, zeros, period / 100, coordinates,
+                          effects=abi.EFFECT_GENERAL_RELATIVITY)
+    return spec, 100000, dict(M=M, m=m, a=a, mu=mu, n=n, period=period, axis=axis)

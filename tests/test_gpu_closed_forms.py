@@ -260,3 +260,11 @@ def test_planet_flattening_case_and_adjoint(oracle, a, e):
     vjp = g["fluid_love_number"][1]
     print(f"dA/dk2f_planet: adjoint {vjp:.12e}, tangent {jvp:.12e}", flush=True)
     assert abs(vjp / jvp - 1) < 1e-6
+
+
+@pytest.mark.parametrize("a", [0.02, 0.05])
+def test_spin_orbit_case(oracle, a):
+    """The spin-orbit system (a star spinning about an axis tilted from the
+    orbit normal, Kidder GR alone) on the device, bitwise against the
+    oracle, whose nodal precession tests/test_oracle_closed_forms.py checks."""
+    run_both(oracle, *cases.spin_orbit_case(a)[:2])

@@ -32,7 +32,12 @@ independent integration of Hut's force and torque.
 
 The two other GR implementations (Anderson et al. 1975, Newhall et al. 1983)
 give Einstein's advance, and the wind's magnetic braking (wind.rs:71-100)
-follows its closed-form spin-down, saturated and not.
+follows its closed-form spin-down, saturated and not.  The conservative
+tide gives Sterne's apsidal rate, the star's dissipation the closed-form
+orbital decay, the planet's own bulge its J2 advance, and Kidder's
+spin-orbit term (as the reference writes it, without 1/r^3) a^3 times the
+Lense-Thirring precession.  The oracle's tangent-linear derivatives are
+checked against the closed-form derivatives of several of these.
 """
 import numpy as np
 import pytest
@@ -579,3 +584,39 @@ def test_planet_flattening_apsidal_advance(oracle, a, e):
     np.testing.assert_array_equal(st["spin"][1], spec.body["spin"][1])   # no torque
     dA = gr @ (tan["x"][0, 1] - tan["x"][0, 0]) + gv @ (tan["v"][0, 1] - tan["v"][0, 0])
     assert abs(dA / ((A - A0) / P["k2f"]) - 1) < 1e-5
+
+
+def node_longitude(x, v, axis):
+    """Longitude of the orbit normal's projection on the plane perpendicular
+    to `axis`."""
+    h = np.cross(x[1] - x[0], v[1] - v[0])
+    h /= np.linalg.norm(h)
+    hp = h - (h @ axis) * axis
+    e1 = np.cross([0.0, 1.0, 0.0], axis)
+    e1 /= np.linalg.norm(e1)
+    return np.arctan2(hp @ np.cross(axis, e1), hp @ e1)
+
+
+@pytest.mark.parametrize("a", [0.02, 0.05])
+def test_kidder_spin_orbit_nodal_precession(oracle, a):
+    """The reference's spin-orbit term (general_relativity.rs:300-456,
+    Kidder 1995 equations 2.2c, 2.4a and 2.4b) multiplies each bracket by G /
+    c^2 (`factor_a`, line 368) where Kidder has G / (c^2 r^3): as written the
+    spin-orbit acceleration and spin torques are r^3 (in AU) times Kidder's.
+    For a planet of negligible mass on a circular orbit Kidder's term
+    precesses the orbit normal about the star's spin at the Lense-Thirring
+    rate 2 G S / (c^2 a^3); the reference's equations therefore give a^3
+    times it.  The oracle keeps the reference's equations (the device
+    reproduces them bit for bit), so its node advance over 1e5 steps must be
+    a^3 times Lense-Thirring's (measured 1 + 3e-5 of it at 0.02 AU)."""
+    spec, steps, P = cases.spin_orbit_case(a)
+    out = oracle.run(spec, steps)
+    assert _ok(out["status"][0])
+    S = spec.body["angular_momentum"][0]
+    axis = S / np.linalg.norm(S)
+    d = node_longitude(out["position"], out["velocity"], axis) - \
+        node_longitude(spec.body["position"], spec.body["velocity"], axis)
+    d = (d + np.pi) % (2 * np.pi) - np.pi
+    lense_thirring = 2 * gen.G * np.linalg.norm(S) / (C2 * a ** 3) * out["current_time"][0]
+    print(f"a={a}: node advance {d:.6e} rad, Lense-Thirring {lense_thirring:.6e}, ratio {d / lense_thirring:.6e}")
+    assert abs(d / (a ** 3 * lense_thirring) - 1) < 1e-3
