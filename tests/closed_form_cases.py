@@ -142,6 +142,11 @@ def wind_case(omega0, effects, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
                              rate=k * np.sqrt(R / gen.R_SUN / M) / inertia)
 
 
+# (M, m, R, a, e) of the conservative-tide systems: an Earth around an M
+# dwarf, a Neptune-mass planet on an eccentric orbit around a Sun
+TIDAL_LOVE_CASES = [(0.08, gen.M_EARTH, gen.R_EARTH, 0.01, 0.1), (1.0, 1e-4, 0.05 * gen.R_SUN, 0.04, 0.3)]
+
+
 def tidal_love_case(M, m, R, a, e, coordinates=abi.COORD_DEMOCRATIC_HELIOCENTRIC):
     """Only tides on with zero dissipation: the conservative tide the star
     raises on the planet (Love number 0.3; the star's is zero), the spins

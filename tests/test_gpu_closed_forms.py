@@ -75,6 +75,10 @@ OTHER_COORDINATE_CASES = {
     "tides_spin_up": lambda c: cases.ctl_case(0.2, c),
     "wind_alone": lambda c: cases.wind_case(3.0, abi.EFFECT_WIND, c),
     "wind_across_saturation": lambda c: cases.wind_case(3.0, abi.EFFECT_WIND | abi.EFFECT_TIDES, c),
+    "conservative_tide": lambda c: cases.tidal_love_case(*cases.TIDAL_LOVE_CASES[0], coordinates=c),
+    "stellar_tide_decay": lambda c: cases.decay_case(1e3, c),
+    "planet_flattening": lambda c: cases.planet_flattening_case(0.01, 0.1, c),
+    "spin_orbit": lambda c: cases.spin_orbit_case(0.02, c),
 }
 
 
@@ -188,8 +192,8 @@ def test_conservative_tide_case_and_adjoint(oracle, case):
     the planet's Love number against the oracle's tangent-linear derivative
     (1e-6); tests/test_oracle_closed_forms.py checks both against the closed
     form."""
-    from tests.test_oracle_closed_forms import TIDAL_LOVE_CASES, advance_gradient
-    spec, steps, P = cases.tidal_love_case(*TIDAL_LOVE_CASES[case])
+    from tests.test_oracle_closed_forms import advance_gradient
+    spec, steps, P = cases.tidal_love_case(*cases.TIDAL_LOVE_CASES[case])
     out = run_both(oracle, spec, steps)
     nb = spec.n_bodies
     A, gr, gv = advance_gradient(spec, out, P["mu"])

@@ -471,7 +471,7 @@ def test_tangent_linear_flattening_advance_derivative(oracle, a, e):
     assert abs(dA / (A / P["k2f"]) - 1) < 2e-5
 
 
-TIDAL_LOVE_CASES = [(0.08, gen.M_EARTH, gen.R_EARTH, 0.01, 0.1), (1.0, 1e-4, 0.05 * gen.R_SUN, 0.04, 0.3)]
+TIDAL_LOVE_CASES = cases.TIDAL_LOVE_CASES
 
 
 def tidal_love_advance(P, t):
@@ -619,4 +619,50 @@ def test_kidder_spin_orbit_nodal_precession(oracle, a):
     d = (d + np.pi) % (2 * np.pi) - np.pi
     lense_thirring = 2 * gen.G * np.linalg.norm(S) / (C2 * a ** 3) * out["current_time"][0]
     print(f"a={a}: node advance {d:.6e} rad, Lense-Thirring {lense_thirring:.6e}, ratio {d / lense_thirring:.6e}")
+    assert abs(d / (a ** 3 * lense_thirring) - 1) < 1e-3
+
+
+@pytest.mark.parametrize("coordinates", ["Jacobi", "WHDS"])
+def test_more_closed_forms_in_jacobi_and_whds_coordinates(oracle, coordinates):
+    """The conservative tide, the stellar-tide decay, the planet's bulge and
+    the spin-orbit precession integrated in Jacobi and WHDS coordinates meet
+    the bounds they meet in democratic heliocentric coordinates (for two
+    bodies these splittings leave the Kepler orbit's own precession at 1e-13
+    rad, against 1e-8 in democratic heliocentric coordinates)."""
+    c = abi.COORDINATES[coordinates]
+    spec, steps, P = cases.tidal_love_case(*TIDAL_LOVE_CASES[0], coordinates=c)
+    st = oracle.run(spec, steps)
+    assert _ok(st["status"][0])
+    A, _, _ = advance_gradient(spec, st, P["mu"])
+    assert abs(A / tidal_love_advance(P, st["current_time"][0]) - 1) < 2e-3
+
+    spec, steps, P = cases.decay_case(1e3, c)
+    st = oracle.run(spec, steps)
+    assert _ok(st["status"][0])
+    a1, _, _ = semi_major_axis_and_gradient(st, P["mu"])
+    closed = (P["a"] ** 8 - 48 * gen.G * P["m"] * P["k2dt"] * P["Rs"] ** 5 * st["current_time"][0]) ** 0.125
+    assert abs((a1 - P["a"]) / (closed - P["a"]) - 1) < 2e-4
+
+    a, e = 0.01, 0.1
+    spec, steps, P = cases.planet_flattening_case(a, e, c)
+    st = oracle.run(spec, steps)
+    assert _ok(st["status"][0])
+    A, _, _ = advance_gradient(spec, st, P["mu"])
+    spec0, _, _ = cases.planet_flattening_case(a, e, c)
+    spec0.body["flattening_love_number"][1] = 0.0
+    A0, _, _ = advance_gradient(spec0, oracle.run(spec0, steps), P["mu"])
+    J2 = P["k2f"] * P["omega"] ** 2 * P["R"] ** 3 / (3 * gen.G * P["m"])
+    closed = 1.5 * P["n"] * J2 * (P["R"] / (a * (1 - e ** 2))) ** 2 * st["current_time"][0]
+    assert abs((A - A0) / closed - 1) < 1e-5
+
+    a = 0.02
+    spec, steps, P = cases.spin_orbit_case(a, c)
+    st = oracle.run(spec, steps)
+    assert _ok(st["status"][0])
+    S = spec.body["angular_momentum"][0]
+    axis = S / np.linalg.norm(S)
+    d = node_longitude(st["position"], st["velocity"], axis) - \
+        node_longitude(spec.body["position"], spec.body["velocity"], axis)
+    d = (d + np.pi) % (2 * np.pi) - np.pi
+    lense_thirring = 2 * gen.G * np.linalg.norm(S) / (C2 * a ** 3) * st["current_time"][0]
     assert abs(d / (a ** 3 * lense_thirring) - 1) < 1e-3
