@@ -245,8 +245,9 @@ def test_constant_time_lag_spin_relaxation(oracle, spin_over_n):
 @pytest.mark.parametrize("a,e,M", [(0.05, 0.1, 1.0), (0.03, 0.3, 1.0), (0.05, 0.1, 0.5)])
 def test_other_general_relativity_implementations_give_einsteins_advance(oracle, implementation, a, e, M):
     """The reference's two other GR implementations are first post-Newtonian
-    (general_relativity.rs:299-456: Anderson et al. 1975 for the host's mass
-    alone, Newhall et al. 1983 for every pair); for a planet of negligible
+    (Anderson et al. 1975 for the host's mass alone,
+    general_relativity.rs:461-682; Newhall et al. 1983 for every pair,
+    683-898); for a planet of negligible
     mass both advance the periapsis by Einstein's 6 pi G M / (c^2 a (1 - e^2))
     per orbit, whatever their gauge (measured: 1e-5 to 2.4e-5 of it over 300
     orbits)."""
